@@ -1,0 +1,136 @@
+#!/usr/bin/env python3
+"""Headline benchmark: synaptic events/s and ms per simulated second of one
+N=100,000, Nsyn=1000 Izhikevich network (balanced regime) on one MI355X.
+
+A "step" is one 1 ms simulation step of the whole network: Izhikevich update
+of every neuron, spike compaction, propagation of the step's spikes.  The
+network (1e8 synapses, 800 MB) is resident in HBM before the timed region.
+
+Prints ONE JSON line (rank 0).  Extra lines (per-scheme sweep) go to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def step_bytes(n, nsyn, spikes):
+    """Algorithmic bytes: 40 B per neuron-step (read v,u,a,b,c,d,Isyn; write
+    v,u,Isyn) + per spike 4 B queue entry + 8 B per synaptic event."""
+    return 40.0 * n + (4.0 + 8.0 * nsyn) * spikes
+
+
+def cpu_baseline(n, nsyn, seed, regime, steps):
+    """The CPU oracle (single thread, fp32, -O2 -ffp-contract=off) on the first
+    `steps` steps of the same network; connectivity build excluded."""
+    from oracle import pyoracle
+    net = pyoracle.Network(n, nsyn, seed, regime=regime)
+    t0 = time.perf_counter()
+    _, counts, _ = net.run(steps, raster=False)
+    dt = time.perf_counter() - t0
+    ev = float(counts.sum()) * nsyn
+    return {"value": ev / dt, "unit": "synaptic events/s", "cores": 1, "kind": "port",
+            "sample": f"first {steps} steps of the same N={n} Nsyn={nsyn} network "
+                      f"({int(counts.sum())} spikes, {dt:.2f} s, single-thread C oracle)",
+            "ms_per_sim_second": dt * 1000.0 / steps * 1000.0}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--n", type=int, default=100_000)
+    ap.add_argument("--nsyn", type=int, default=1000)
+    ap.add_argument("--seed", type=int, default=20221)
+    ap.add_argument("--regime", type=int, default=1)
+    ap.add_argument("--alg", default="SKL")
+    ap.add_argument("--accum", default="atomic")
+    ap.add_argument("--sweep", action="store_true", help="also time every scheme x mode (stderr)")
+    ap.add_argument("--cpu-steps", type=int, default=300)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(independent networks shard by placement; see DESIGN.md). Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import gpu4snn_amd as g
+
+    t_build = time.perf_counter()
+    net = g.Network(args.n, args.nsyn, args.seed, regime=args.regime)
+    t_build = time.perf_counter() - t_build
+    print(f"network N={args.n} Nsyn={args.nsyn} built+uploaded in {t_build:.1f} s", file=sys.stderr, flush=True)
+
+    if args.sweep:
+        for accum in ("atomic", "det"):
+            for alg in ("AP", "N", "S", "AB", "SKL"):
+                net.reset()
+                net.run(args.warmup, alg=alg, accum=accum, raster=False)
+                _, c = net.run(args.steps, alg=alg, accum=accum, raster=False)
+                tm = net.timing()
+                ms = tm["device_ms"] / args.steps
+                by = sum(step_bytes(args.n, args.nsyn, int(x)) for x in c)
+                print(f"sweep {alg:>3}/{accum:<6} {ms*1000:8.2f} us/step  {ms*1000:8.1f} ms/sim-s  "
+                      f"{c.sum()*args.nsyn/(tm['device_ms']/1e3):.3e} ev/s  "
+                      f"{by/(tm['device_ms']/1e3)/1e9:8.1f} GB/s  spikes/step {c.mean():.0f}",
+                      file=sys.stderr, flush=True)
+
+    net.reset()
+    net.run(args.warmup, alg=args.alg, accum=args.accum, raster=False)
+    t0 = time.perf_counter()
+    _, counts = net.run(args.steps, alg=args.alg, accum=args.accum, raster=False)
+    wall = time.perf_counter() - t0
+    tm = net.timing()
+    spikes = int(counts.sum())
+    events = spikes * args.nsyn
+    dev_s = tm["device_ms"] / 1e3
+    total_bytes = sum(step_bytes(args.n, args.nsyn, int(x)) for x in counts)
+    launches = max(1, tm["launches"])
+    per_launch_bytes = total_bytes / launches
+    avg_launch_s = dev_s / launches
+    achieved = per_launch_bytes / avg_launch_s / 1e9
+
+    out = {
+        "metric": "synaptic events/s + ms per sim-second, N=100k; p50/p99 req latency (served)",
+        "value": events / wall,
+        "unit": "synaptic events/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "ms_per_sim_second": wall * 1e3 / args.steps * 1000.0,
+        "device_ms_per_step": tm["device_ms"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Philox network and thalamic stream)",
+        "config": {"workload": f"single network N={args.n} Nsyn={args.nsyn} regime={args.regime} "
+                               f"({'quiet balanced irregular'.split()[args.regime]}), 1 ms steps",
+                   "scheme": args.alg, "accum": args.accum, "spikes_per_step": spikes / args.steps,
+                   "firing_rate_hz": spikes / args.steps / args.n * 1000.0, "seed": args.seed},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": {"SKL": "k_skl" if args.accum == "atomic" else "k_skl_det"}.get(args.alg, "step"),
+                     "bytes_per_launch": per_launch_bytes, "avg_launch_ms": avg_launch_s * 1e3},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.n, args.nsyn, args.seed, args.regime, args.cpu_steps)
+        out["cpu_baseline"]["cores"] = 1
+    net.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,203 @@
+"""gpu4snn_amd -- MI355X (gfx950) spike-propagation hot path of GPU4SNN.
+
+Python view of the C ABI in include/gpu4snn.h (libgpu4snn.so, built in-tree
+by gpu4snn_amd/build.py).  The library is required: there is no CPU fallback,
+and every call fails loudly if the shared object or the GPU is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+__all__ = ["ALG", "ACCUM", "Network", "lib", "G4SError", "words_of", "raster_to_dense", "n_exc_of"]
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libgpu4snn.so")
+CLI_PATH = os.path.join(_PKG, "bin", "gpu4snn")
+
+ALG = {"AP": 0, "N": 1, "S": 2, "AB": 3, "SKL": 4}
+ALG_NAMES = {v: k for k, v in ALG.items()}
+ACCUM = {"atomic": 0, "det": 1}
+
+_lib = None
+
+
+class G4SError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"gpu4snn error {code}: {msg}")
+        self.code = code
+
+
+class Spec(C.Structure):
+    _fields_ = [("n", C.c_int64), ("nsyn", C.c_int32), ("regime", C.c_int32), ("seed", C.c_uint64),
+                ("thal_scale", C.c_float), ("reserved", C.c_int32), ("n_exc", C.c_int64)]
+
+
+class RunOpts(C.Structure):
+    _fields_ = [("block", C.c_int32), ("grid_waves", C.c_int32), ("chunk", C.c_int32), ("graph", C.c_int32)]
+
+
+class Timing(C.Structure):
+    _fields_ = [("device_ms", C.c_double), ("wall_ms", C.c_double), ("steps", C.c_int64),
+                ("spikes", C.c_int64), ("launches", C.c_int32), ("grid", C.c_int32)]
+
+
+_vp = C.c_void_p
+
+
+def lib():
+    """Load libgpu4snn.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise G4SError(-2, f"{LIB_PATH} not built; run `python -c 'import __graft_entry__ as g; g.build()'`")
+        L = C.CDLL(LIB_PATH)
+        sigs = {
+            "g4s_create": ([C.POINTER(Spec), C.POINTER(_vp)], C.c_int),
+            "g4s_reset": ([_vp], C.c_int),
+            "g4s_run": ([_vp, C.c_int, C.c_int, C.c_int32, _vp, _vp, C.POINTER(RunOpts)], C.c_int),
+            "g4s_get_timing": ([_vp, C.POINTER(Timing)], C.c_int),
+            "g4s_current_step": ([_vp], C.c_int64),
+            "g4s_get_state": ([_vp, _vp, _vp, _vp, _vp], C.c_int),
+            "g4s_get_params": ([_vp, _vp, _vp, _vp, _vp], C.c_int),
+            "g4s_set_state": ([_vp, _vp, _vp], C.c_int),
+            "g4s_get_synapses": ([_vp, _vp, _vp], C.c_int),
+            "g4s_destroy": ([_vp], C.c_int),
+            "g4s_last_error": ([], C.c_char_p),
+            "g4s_version": ([], C.c_int),
+            "g4s_build_synapses_host": ([C.POINTER(Spec), _vp, _vp, C.c_int32], C.c_int),
+            "g4s_device_thalamic": ([C.c_uint64, C.c_uint32, C.c_int64, _vp], C.c_int),
+            "g4s_device_normal_words": ([_vp, _vp, C.c_int64, _vp], C.c_int),
+        }
+        for name, (args, res) in sigs.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise G4SError(rc, lib().g4s_last_error().decode())
+    return rc
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def n_exc_of(n):
+    """floor(4N/5): the reference's ceil(4*N/5) of an integer quotient (gpu4snn.cu:744)."""
+    return (4 * int(n)) // 5
+
+
+def words_of(n):
+    return (int(n) + 31) // 32
+
+
+def raster_to_dense(words, n):
+    """[steps, ceil(n/32)] uint32 bitmap rows -> [steps, n] uint8 0/1."""
+    w = np.ascontiguousarray(words, np.uint32)
+    bits = np.unpackbits(w.view(np.uint8).reshape(w.shape[0], -1), axis=1, bitorder="little")
+    return bits[:, :n]
+
+
+def make_spec(n, nsyn, seed, regime=1, thal_scale=1.0, n_exc=-1):
+    return Spec(int(n), int(nsyn), int(regime), int(seed) & (2**64 - 1), float(thal_scale), 0, int(n_exc))
+
+
+def build_synapses_host(n, nsyn, seed, regime=1, thal_scale=1.0, threads=0):
+    """Connectivity table built on the host without a GPU (same as g4s_create uploads)."""
+    spec = make_spec(n, nsyn, seed, regime, thal_scale)
+    post = np.empty(int(n) * int(nsyn), np.uint32)
+    w = np.empty(int(n) * int(nsyn), np.float32)
+    _check(lib().g4s_build_synapses_host(C.byref(spec), _ptr(post), _ptr(w), threads))
+    return post, w
+
+
+def device_thalamic(seed, t, n):
+    out = np.empty(int(n), np.float32)
+    _check(lib().g4s_device_thalamic(seed, t, n, _ptr(out)))
+    return out
+
+
+def device_normal_words(x0, x1):
+    x0 = np.ascontiguousarray(x0, np.uint32)
+    x1 = np.ascontiguousarray(x1, np.uint32)
+    out = np.empty(x0.size, np.float32)
+    _check(lib().g4s_device_normal_words(_ptr(x0), _ptr(x1), x0.size, _ptr(out)))
+    return out
+
+
+class Network:
+    """One device-resident network (g4s_create .. g4s_destroy)."""
+
+    def __init__(self, n, nsyn, seed, regime=1, thal_scale=1.0, n_exc=-1):
+        self.n, self.nsyn, self.seed = int(n), int(nsyn), int(seed)
+        self.spec = make_spec(n, nsyn, seed, regime, thal_scale, n_exc)
+        self._h = _vp()
+        _check(lib().g4s_create(C.byref(self.spec), C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().g4s_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def t(self):
+        return lib().g4s_current_step(self._h)
+
+    def reset(self):
+        _check(lib().g4s_reset(self._h))
+
+    def run(self, steps, alg="SKL", accum="atomic", raster=True, block=0, chunk=0, graph=0):
+        """Simulate `steps` steps; returns (raster words [steps, words] or None, counts)."""
+        a = ALG[alg] if isinstance(alg, str) else int(alg)
+        m = ACCUM[accum] if isinstance(accum, str) else int(accum)
+        r = np.zeros((steps, words_of(self.n)), np.uint32) if raster else None
+        counts = np.zeros(steps, np.int32)
+        opts = RunOpts(block, 0, chunk, graph)
+        _check(lib().g4s_run(self._h, a, m, int(steps), _ptr(r), _ptr(counts), C.byref(opts)))
+        return r, counts
+
+    def timing(self):
+        t = Timing()
+        _check(lib().g4s_get_timing(self._h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in Timing._fields_}
+
+    def state(self):
+        v, u, I = (np.empty(self.n, np.float32) for _ in range(3))
+        nos = np.empty(self.n, np.int32)
+        _check(lib().g4s_get_state(self._h, _ptr(v), _ptr(u), _ptr(I), _ptr(nos)))
+        return v, u, I, nos
+
+    def params(self):
+        a, b, c, d = (np.empty(self.n, np.float32) for _ in range(4))
+        _check(lib().g4s_get_params(self._h, _ptr(a), _ptr(b), _ptr(c), _ptr(d)))
+        return a, b, c, d
+
+    def set_state(self, v=None, u=None):
+        v = None if v is None else np.ascontiguousarray(v, np.float32)
+        u = None if u is None else np.ascontiguousarray(u, np.float32)
+        _check(lib().g4s_set_state(self._h, _ptr(v), _ptr(u)))
+
+    def synapses(self):
+        post = np.empty(self.n * self.nsyn, np.uint32)
+        w = np.empty(self.n * self.nsyn, np.float32)
+        _check(lib().g4s_get_synapses(self._h, _ptr(post), _ptr(w)))
+        return post, w

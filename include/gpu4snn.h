@@ -1,0 +1,131 @@
+/*
+ * gpu4snn.h -- C ABI of the MI355X (gfx950) GPU4SNN spike-propagation library
+ * (gpu4snn_amd/libgpu4snn.so).
+ *
+ * The reference (Crypto-TII/GPU4SNN, gpu4snn.cu; "cu:L" below) has no library
+ * API: its hot path is a set of kernels called from main().  Each entry point
+ * here names the reference code it replaces.  Conventions:
+ *   - every function returns G4S_OK (0) or a negative G4S_ERR_* code and
+ *     sets a thread-local message readable with g4s_last_error();
+ *   - the library owns all device memory; callers own every host buffer;
+ *   - a handle must not be used from two threads at once;
+ *   - all device work runs on the library's own HIP stream of the handle.
+ */
+#ifndef GPU4SNN_H
+#define GPU4SNN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define G4S_OK               0
+#define G4S_ERR_ARG         -1  /* invalid argument (N <= 1, Nsyn >= N, unknown alg ...) */
+#define G4S_ERR_HIP         -2  /* HIP runtime error (no device, launch failure ...)     */
+#define G4S_ERR_MEM         -3  /* device or host allocation failed                      */
+#define G4S_ERR_STATE       -4  /* call not valid in the handle's state                  */
+#define G4S_ERR_UNSUPPORTED -5  /* size outside what the selected scheme supports        */
+#define G4S_ERR_TIMEOUT     -6  /* a bounded device-side wait expired; run aborted       */
+#define G4S_ERR_BUDGET      -7  /* serving: request exceeds the device-memory budget      */
+
+/* Propagation schemes: the reference's `dynamic` selector (cu:1083, cu:1230). */
+#define G4S_ALG_AP  0  /* per-spike work items (reference: child grid per spike, cu:408-409) */
+#define G4S_ALG_N   1  /* neuron-parallel (reference deliverspks1, cu:420-443)            */
+#define G4S_ALG_S   2  /* synapse-parallel (reference deliverspks2, cu:446-467)           */
+#define G4S_ALG_AB  3  /* persistent cooperative propagation (AB_deliverspks, cu:554-584)  */
+#define G4S_ALG_SKL 4  /* whole run in one cooperative launch (SKL_deliverspks, cu:669-716) */
+
+/* Accumulation modes of the synaptic input. */
+#define G4S_ACCUM_ATOMIC 0 /* fp32 atomic scatter, any order: the reference's semantics   */
+#define G4S_ACCUM_DET    1 /* ordered gather: each input summed in ascending presynaptic id */
+
+/* Network specification.  Replaces main()'s argv + hard-coded constants
+ * (cu:738-745, cu:362-374, cu:876-899). */
+typedef struct g4s_spec {
+    int64_t  n;          /* neurons N (cu:738)                                     */
+    int32_t  nsyn;       /* outgoing synapses per neuron N_syn (cu:739), < N        */
+    int32_t  regime;     /* 0 quiet, 1 balanced, 2 irregular (cu:362-374)           */
+    uint64_t seed;       /* key of every Philox stream of this network              */
+    float    thal_scale; /* multiplies the regime gains; 1.0 reproduces cu:362-374  */
+    int32_t  reserved;   /* must be 0                                               */
+    int64_t  n_exc;      /* excitatory neurons; < 0 selects floor(4N/5) (cu:744)    */
+} g4s_spec;
+
+typedef struct g4s_net g4s_net;
+
+/* Tuning of one run; zero-initialised fields select the defaults. */
+typedef struct g4s_run_opts {
+    int32_t block;       /* threads per workgroup (multiple of 64), 0 = 256            */
+    int32_t grid_waves;  /* N/S schemes: waves launched per step, 0 = from last count  */
+    int32_t chunk;       /* steps per device chunk between host copies, 0 = 1000       */
+    int32_t graph;       /* 1 = replay per-step schemes from a captured HIP graph      */
+} g4s_run_opts;
+
+/* Timing of the last g4s_run call. */
+typedef struct g4s_timing {
+    double  device_ms;   /* HIP-event time of the step loop, raster copies excluded    */
+    double  wall_ms;     /* host steady-clock time of the whole call                   */
+    int64_t steps;       /* steps simulated                                            */
+    int64_t spikes;      /* spikes in those steps                                      */
+    int32_t launches;    /* kernel launches issued                                     */
+    int32_t grid;        /* workgroups of the dominant kernel                          */
+} g4s_timing;
+
+/* Allocate device memory, build the connectivity table, initialise neurons
+ * and zero the synaptic input.  Replaces cu:753-777 (allocation), cu:862-912
+ * (host connectivity sampler + upload) and cu:1094 (initNeuron). */
+int g4s_create(const g4s_spec *spec, g4s_net **out);
+
+/* Re-initialise neuron parameters/state (initNeuron, cu:286-320), zero the
+ * synaptic input and spike flags (fixes cu:776-777), and set step t = 0. */
+int g4s_reset(g4s_net *net);
+
+/* Simulate `steps` steps with scheme `alg` and accumulation mode `accum`,
+ * continuing from the handle's current step.  Replaces the step loop
+ * cu:1215-1271 (AP/N/S/AB) and the SKL launch cu:1282-1284.
+ *   raster : NULL or steps * ceil(N/32) words; bit (i & 31) of word
+ *            [s * ceil(N/32) + (i >> 5)] is set when neuron i spiked at step
+ *            t0 + s (the reference's spiketimes rows, cu:1256-1261)
+ *   counts : NULL or `steps` spike counts (SpksPerStep, cu:1267)
+ *   opts   : NULL for defaults */
+int g4s_run(g4s_net *net, int alg, int accum, int32_t steps,
+            uint32_t *raster, int32_t *counts, const g4s_run_opts *opts);
+
+int g4s_get_timing(const g4s_net *net, g4s_timing *out);
+
+/* Current step index (number of steps simulated since the last reset). */
+int64_t g4s_current_step(const g4s_net *net);
+
+/* Copy the structure-of-arrays state out (any pointer may be NULL).  I is the
+ * total input of the last simulated step; replaces the AoS Neuron copy-back
+ * behind neuroninfo(after).csv (cu:1099, cu:1318-1327). */
+int g4s_get_state(g4s_net *net, float *v, float *u, float *I, int32_t *nospks);
+int g4s_get_params(g4s_net *net, float *a, float *b, float *c, float *d);
+/* Overwrite v and u (NULL leaves one unchanged), e.g. to inject a NaN. */
+int g4s_set_state(g4s_net *net, const float *v, const float *u);
+
+/* Copy the connectivity table: row i holds the Nsyn targets of neuron i in
+ * ascending order (cu:895-896 layout [N][N_syn]) and their weights. */
+int g4s_get_synapses(g4s_net *net, uint32_t *post, float *w);
+
+/* Free every device allocation of the handle (cu:1331 cudaDeviceReset). */
+int g4s_destroy(g4s_net *net);
+
+const char *g4s_last_error(void);
+int g4s_version(void);
+
+/* Host-only: the connectivity builder without a device (same table as
+ * g4s_create uploads).  `threads` <= 0 uses the hardware concurrency. */
+int g4s_build_synapses_host(const g4s_spec *spec, uint32_t *post, float *w, int32_t threads);
+
+/* Device diagnostics: thalamic draws of neurons [0, n) at step t under
+ * `seed` computed by a gfx950 kernel, and the Box-Muller transform of given
+ * word pairs on the device. */
+int g4s_device_thalamic(uint64_t seed, uint32_t t, int64_t n, float *out);
+int g4s_device_normal_words(const uint32_t *x0, const uint32_t *x1, int64_t n, float *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
