@@ -21,6 +21,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
+# dominant kernel of each (scheme, mode): the one launch that carries a step
+KERNEL_OF = {("SKL", "atomic"): "k_skl_slice", ("AB", "atomic"): "k_slice_step", ("AP", "atomic"): "k_ap",
+             ("SKL", "det"): "k_skl_det", ("AB", "det"): "k_pull_thread", ("N", "det"): "k_pull_thread",
+             ("S", "det"): "k_pull_wave", ("AP", "det"): "k_ap_det"}
+
 
 def step_bytes(n, nsyn, spikes):
     """Algorithmic bytes: 40 B per neuron-step (read v,u,a,b,c,d,Isyn; write
@@ -52,8 +57,10 @@ def main():
     ap.add_argument("--nsyn", type=int, default=1000)
     ap.add_argument("--seed", type=int, default=20221)
     ap.add_argument("--regime", type=int, default=1)
-    ap.add_argument("--alg", default="SKL")
+    ap.add_argument("--alg", default="auto", help="scheme, or 'auto' for the autotuner's pick")
     ap.add_argument("--accum", default="atomic")
+    ap.add_argument("--slices", type=int, default=0)
+    ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--sweep", action="store_true", help="also time every scheme x mode (stderr)")
     ap.add_argument("--cpu-steps", type=int, default=300)
     ap.add_argument("--no-cpu", action="store_true")
@@ -75,6 +82,7 @@ def main():
     if args.sweep:
         for accum in ("atomic", "det"):
             for alg in ("AP", "N", "S", "AB", "SKL"):
+                print(f"sweep {alg}/{accum} ...", file=sys.stderr, flush=True)
                 net.reset()
                 net.run(args.warmup, alg=alg, accum=accum, raster=False)
                 _, c = net.run(args.steps, alg=alg, accum=accum, raster=False)
@@ -86,10 +94,20 @@ def main():
                       f"{by/(tm['device_ms']/1e3)/1e9:8.1f} GB/s  spikes/step {c.mean():.0f}",
                       file=sys.stderr, flush=True)
 
+    plan_info = None
+    if args.alg == "auto":
+        t_tune = time.perf_counter()
+        plan = net.autotune(args.accum, trial_steps=100)
+        plan_info = plan.as_dict()
+        plan_info["tune_s"] = time.perf_counter() - t_tune
+        alg, block, slices = g.ALG_NAMES[plan.alg], plan.opts.block, plan.opts.slices
+        print(f"autotune: {plan_info}", file=sys.stderr, flush=True)
+    else:
+        alg, block, slices = args.alg, args.block, args.slices
     net.reset()
-    net.run(args.warmup, alg=args.alg, accum=args.accum, raster=False)
+    net.run(args.warmup, alg=alg, accum=args.accum, raster=False, block=block, slices=slices)
     t0 = time.perf_counter()
-    _, counts = net.run(args.steps, alg=args.alg, accum=args.accum, raster=False)
+    _, counts = net.run(args.steps, alg=alg, accum=args.accum, raster=False, block=block, slices=slices)
     wall = time.perf_counter() - t0
     tm = net.timing()
     spikes = int(counts.sum())
@@ -118,11 +136,12 @@ def main():
         "data": "synthetic (seeded Philox network and thalamic stream)",
         "config": {"workload": f"single network N={args.n} Nsyn={args.nsyn} regime={args.regime} "
                                f"({'quiet balanced irregular'.split()[args.regime]}), 1 ms steps",
-                   "scheme": args.alg, "accum": args.accum, "spikes_per_step": spikes / args.steps,
+                   "scheme": alg, "accum": args.accum, "block": block, "slices": slices,
+                   "autotune": plan_info, "spikes_per_step": spikes / args.steps,
                    "firing_rate_hz": spikes / args.steps / args.n * 1000.0, "seed": args.seed},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": {"SKL": "k_skl" if args.accum == "atomic" else "k_skl_det"}.get(args.alg, "step"),
+                     "kernel": KERNEL_OF.get((alg, args.accum), "step"),
                      "bytes_per_launch": per_launch_bytes, "avg_launch_ms": avg_launch_s * 1e3},
     }
     if not args.no_cpu:

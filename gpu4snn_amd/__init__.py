@@ -36,7 +36,17 @@ class Spec(C.Structure):
 
 
 class RunOpts(C.Structure):
-    _fields_ = [("block", C.c_int32), ("grid_waves", C.c_int32), ("chunk", C.c_int32), ("graph", C.c_int32)]
+    _fields_ = [("block", C.c_int32), ("slices", C.c_int32), ("chunk", C.c_int32), ("graph", C.c_int32)]
+
+
+class Plan(C.Structure):
+    _fields_ = [("alg", C.c_int32), ("accum", C.c_int32), ("opts", RunOpts), ("us_per_step", C.c_float),
+                ("rate_hz", C.c_float)]
+
+    def as_dict(self):
+        return {"alg": ALG_NAMES[self.alg], "accum": {0: "atomic", 1: "det"}[self.accum],
+                "block": self.opts.block, "slices": self.opts.slices, "us_per_step": self.us_per_step,
+                "rate_hz": self.rate_hz}
 
 
 class Timing(C.Structure):
@@ -57,6 +67,7 @@ def lib():
         sigs = {
             "g4s_create": ([C.POINTER(Spec), C.POINTER(_vp)], C.c_int),
             "g4s_reset": ([_vp], C.c_int),
+            "g4s_set_regime": ([_vp, C.c_int, C.c_float], C.c_int),
             "g4s_run": ([_vp, C.c_int, C.c_int, C.c_int32, _vp, _vp, C.POINTER(RunOpts)], C.c_int),
             "g4s_get_timing": ([_vp, C.POINTER(Timing)], C.c_int),
             "g4s_current_step": ([_vp], C.c_int64),
@@ -70,6 +81,7 @@ def lib():
             "g4s_build_synapses_host": ([C.POINTER(Spec), _vp, _vp, C.c_int32], C.c_int),
             "g4s_device_thalamic": ([C.c_uint64, C.c_uint32, C.c_int64, _vp], C.c_int),
             "g4s_device_normal_words": ([_vp, _vp, C.c_int64, _vp], C.c_int),
+            "g4s_autotune": ([_vp, C.c_int, C.c_int32, C.c_int, C.POINTER(Plan)], C.c_int),
         }
         for name, (args, res) in sigs.items():
             f = getattr(L, name)
@@ -165,15 +177,26 @@ class Network:
     def reset(self):
         _check(lib().g4s_reset(self._h))
 
-    def run(self, steps, alg="SKL", accum="atomic", raster=True, block=0, chunk=0, graph=0):
+    def run(self, steps, alg="SKL", accum="atomic", raster=True, block=0, chunk=0, graph=0, slices=0):
         """Simulate `steps` steps; returns (raster words [steps, words] or None, counts)."""
         a = ALG[alg] if isinstance(alg, str) else int(alg)
         m = ACCUM[accum] if isinstance(accum, str) else int(accum)
         r = np.zeros((steps, words_of(self.n)), np.uint32) if raster else None
         counts = np.zeros(steps, np.int32)
-        opts = RunOpts(block, 0, chunk, graph)
+        opts = RunOpts(block, slices, chunk, graph)
         _check(lib().g4s_run(self._h, a, m, int(steps), _ptr(r), _ptr(counts), C.byref(opts)))
         return r, counts
+
+    def autotune(self, accum="atomic", trial_steps=100, force=False):
+        """Fastest scheme + tiling for this network (g4s_autotune); resets the network."""
+        p = Plan()
+        _check(lib().g4s_autotune(self._h, ACCUM[accum], int(trial_steps), int(bool(force)), C.byref(p)))
+        return p
+
+    def run_plan(self, plan, steps, raster=True, chunk=0):
+        """Run `steps` steps with a plan returned by autotune()."""
+        return self.run(steps, alg=plan.alg, accum=plan.accum, raster=raster, block=plan.opts.block,
+                        chunk=chunk, graph=plan.opts.graph, slices=plan.opts.slices)
 
     def timing(self):
         t = Timing()

@@ -8,7 +8,7 @@ namespace g4s {
 constexpr uint32_t kSentinel = 0xFFFFFFFFu;   // empty spike-queue slot
 constexpr int kCtlStride = 4;                 // per-step control words
 enum { kCtlCount = 0, kCtlHead = 1, kCtlDone = 2 };
-constexpr uint32_t kSpinLimit = 1u << 26;     // bounded device waits (seconds)
+constexpr uint32_t kSpinLimit = 1u << 24;     // bounded device waits (~10 s)
 
 // Everything a kernel needs about one network, passed by value.
 //  - neuron state/parameters are structure-of-arrays fp32 (the reference's
@@ -38,6 +38,17 @@ struct DevNet {
     uint32_t *raster;
     uint32_t *bar;
     int32_t *err;
+    // Slice partition of the targets for the LDS-accumulating schemes: slice
+    // p owns neurons [p*L, min((p+1)*L, n)), L a multiple of 64.  soff[i*(P+1)
+    // + p] is the index in row i of its first target >= p*L (rows ascending).
+    const uint16_t *soff;
+    int32_t P, L;
+    // Per-slice spike counts of each step of the chunk ([chunk][P]), written
+    // without atomics and summed into ctl by k_reduce_counts.
+    int32_t *bcnt;
+    // Per-step kernels replayed from a HIP graph read the chunk's first
+    // step and the run's last step here: tinfo[0] = t0, tinfo[1] = last.
+    const int32_t *tinfo;
 };
 
 struct Launch {
@@ -50,20 +61,31 @@ hipError_t launch_init(const DevNet &d, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, size_t n, uint32_t val, hipStream_t s);
 hipError_t launch_update(const DevNet &d, int t, int rel, int last, const Launch &L);
 hipError_t launch_push_rows(const DevNet &d, int t, int rel, const Launch &L);
-hipError_t launch_push_flat(const DevNet &d, int t, int rel, const Launch &L, bool cooperative);
-hipError_t launch_ap(const DevNet &d, int t, int rel, int last, const Launch &L);
-hipError_t launch_skl(const DevNet &d, int t0, int steps, int last_t, const Launch &L);
+// Scatter every row set in step t's bitmap into isyn[(t+1)&1] (rebuild of
+// the pending input buffer when a global-atomic scheme follows one that
+// keeps none).
+hipError_t launch_push_bitmap(const DevNet &d, int t, hipStream_t s);
+hipError_t launch_reduce_counts(const DevNet &d, int steps, hipStream_t s);
+hipError_t launch_push_flat(const DevNet &d, int t, int rel, const Launch &L);
+hipError_t launch_ap(const DevNet &d, int t, int rel, int last, int prev_count, const Launch &L);
+hipError_t launch_slice_offsets(const DevNet &d, uint16_t *soff, hipStream_t s);
+// AB step `rel` of the chunk; the absolute step is tinfo[0] + rel.
+hipError_t launch_slice_step(const DevNet &d, int rel, const Launch &L);
+hipError_t launch_skl_slice(const DevNet &d, int t0, int steps, int last_t, const Launch &L);
+// Co-resident workgroups per CU of the SKL slice kernel used for slice
+// length L at `block` threads.
+int skl_slice_blocks_per_cu(int L, int block);
 hipError_t launch_pull_thread(const DevNet &d, int t, int rel, int last, const Launch &L, bool cooperative);
 hipError_t launch_pull_wave(const DevNet &d, int t, int rel, int last, const Launch &L);
 hipError_t launch_ap_det(const DevNet &d, int t, int rel, int last, const Launch &L);
 hipError_t launch_skl_det(const DevNet &d, int t0, int steps, int last_t, const Launch &L);
-hipError_t launch_gather_isyn(const DevNet &d, int t, hipStream_t s);
 hipError_t launch_thalamic(uint64_t seed, uint32_t t, int64_t n, float *out, hipStream_t s);
 hipError_t launch_normal_words(const uint32_t *x0, const uint32_t *x1, int64_t n, float *out,
                                hipStream_t s);
 
 // Co-resident workgroups of each cooperative kernel at `block` threads.
-enum CoopKernel { kCoopPushFlat, kCoopAP, kCoopSKL, kCoopPull, kCoopAPDet, kCoopSKLDet };
-int coop_max_blocks(CoopKernel k, int block);
+enum CoopKernel { kCoopAP, kCoopSKLSlice, kCoopPull, kCoopAPDet, kCoopSKLDet };
+int coop_blocks_per_cu(CoopKernel k, int block, size_t shmem);
+int device_cus();
 
 }  // namespace g4s

@@ -65,11 +65,15 @@ __device__ __forceinline__ bool neuron_step(const DevNet &d, int t, int j, float
 // Spike output of one wave covering neurons [wave_base, wave_base+64):
 // two bitmap words (always written, so stale bits are cleared), the raster
 // row of the chunk, and the queue entries appended at one atomic per wave.
+// Waves starting at or beyond `limit` (the end of the caller's neuron range)
+// write nothing: their words belong to another workgroup.
 // kPublish: queue slots are stored with agent-scope atomics because
 // consumers of the same launch read them (AP scheme).
 template <bool kPublish>
-__device__ __forceinline__ void emit(const DevNet &d, int t, int rel, int wave_base, bool spk, int j)
+__device__ __forceinline__ void emit(const DevNet &d, int t, int rel, int wave_base, bool spk, int j,
+                                     int limit)
 {
+    if (wave_base >= limit) return;  // uniform over the wave
     const uint64_t mask = __ballot(spk);
     const int lane = lane_id();
     uint32_t *bm = (t & 1) ? d.bm1 : d.bm0;
@@ -136,9 +140,11 @@ __device__ __forceinline__ float gather_input(const DevNet &d, int t, int j)
     return acc;
 }
 
-// One wave scatters row s of the outgoing table into `out`.
+// One wave scatters row s of the outgoing table into `out`.  Queue entries
+// outside [0, n) (an unpublished kSentinel slot) are ignored.
 __device__ __forceinline__ void push_row(const DevNet &d, uint32_t s, float *out, int lane)
 {
+    if (s >= (uint32_t)d.n) return;
     const uint2 *__restrict__ row = d.syn + (size_t)s * (size_t)d.nsyn;
     const int nsyn = d.nsyn;
     int k = lane;
@@ -168,7 +174,9 @@ __device__ __forceinline__ void push_flat(const DevNet &d, int t, int count, int
         int64_t k = g - qi * d.nsyn;
         if (k < 0) { qi -= 1; k += d.nsyn; }
         else if (k >= d.nsyn) { qi += 1; k -= d.nsyn; }
-        const uint2 e = d.syn[(size_t)q[qi] * (size_t)d.nsyn + (size_t)k];
+        const uint32_t s = q[qi];
+        if (s >= (uint32_t)d.n) continue;
+        const uint2 e = d.syn[(size_t)s * (size_t)d.nsyn + (size_t)k];
         unsafeAtomicAdd(&out[e.x], u2f(e.y));
     }
 }
@@ -187,11 +195,15 @@ __device__ bool grid_sync(const DevNet &d, uint32_t &epoch)
         __hip_atomic_fetch_add(d.bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         int ok = 1;
         uint32_t spins = 0;
-        while (__hip_atomic_load(d.bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        // Relaxed polling: an acquire load would invalidate the caches on
+        // every iteration while other workgroups still compute.  One acquire
+        // fence after the wait orders the reads that follow.
+        while (__hip_atomic_load(d.bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
             if (error_set(d)) { ok = 0; break; }
             if (++spins > kSpinLimit) { raise_error(d); ok = 0; break; }
-            __builtin_amdgcn_s_sleep(1);
+            __builtin_amdgcn_s_sleep(2);
         }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         s_ok = ok;
     }
     __syncthreads();
@@ -237,7 +249,7 @@ __global__ void __launch_bounds__(256) k_update(DevNet d, int t, int rel, int la
         const int j = base + (int)threadIdx.x;
         bool spk = false;
         if (j < d.n) spk = neuron_step(d, t, j, take_input(d, t, j), last != 0);
-        emit<false>(d, t, rel, base + ((int)threadIdx.x & ~(kWave - 1)), spk, j);
+        emit<false>(d, t, rel, base + ((int)threadIdx.x & ~(kWave - 1)), spk, j, d.n);
     }
 }
 
@@ -253,8 +265,27 @@ __global__ void __launch_bounds__(256) k_push_rows(DevNet d, int t, int rel)
         push_row(d, q[r], out, lane);
 }
 
-// (c) S scheme (launched plainly) and AB scheme (launched cooperatively
-// with a persistent, co-resident grid): thread per synapse of queued rows.
+// Scatter every row whose bit is set in step t's bitmap into isyn[(t+1)&1]:
+// a wave per 64 neurons walks its two bitmap words.
+__global__ void __launch_bounds__(256) k_push_bitmap(DevNet d, int t)
+{
+    const uint32_t *bm = (t & 1) ? d.bm1 : d.bm0;
+    float *out = ((t + 1) & 1) ? d.isyn1 : d.isyn0;
+    const int lane = lane_id();
+    const int waves = gridDim.x * (blockDim.x / kWave);
+    const int groups = (d.words + 1) / 2;
+    for (int gi = blockIdx.x * (blockDim.x / kWave) + (int)(threadIdx.x / kWave); gi < groups; gi += waves) {
+        uint64_t m = bm[2 * gi];
+        if (2 * gi + 1 < d.words) m |= (uint64_t)bm[2 * gi + 1] << 32;
+        while (m) {
+            const int k = __builtin_ctzll(m);
+            m &= m - 1;
+            push_row(d, (uint32_t)(gi * 64 + k), out, lane);
+        }
+    }
+}
+
+// (c) S scheme: thread per synapse of queued rows.
 __global__ void __launch_bounds__(256) k_push_flat(DevNet d, int t, int rel)
 {
     const int count = d.ctl[rel * kCtlStride + kCtlCount];
@@ -266,16 +297,24 @@ __global__ void __launch_bounds__(256) k_push_flat(DevNet d, int t, int rel)
 // share of neurons and publishes spikes into the queue, then becomes a
 // consumer that claims queued rows one at a time (atomic head) and
 // scatters them.  A row is claimed as soon as its slot is published, so
-// propagation overlaps the update.  The queue slot holds kSentinel until
-// published and is reset to kSentinel once consumed.
-__global__ void __launch_bounds__(256) k_ap(DevNet d, int t, int rel, int last)
+// propagation overlaps the update.  A slot holds kSentinel until published.
+// Step t re-arms the next step's buffer q[(t+1)&1] (only its first
+// count(t-1) slots can be non-sentinel); nothing reads that buffer during
+// step t, and consumed slots keep their ids so the last step's queue stays
+// valid for whichever scheme runs next.
+__global__ void __launch_bounds__(256) k_ap(DevNet d, int t, int rel, int last, int prev_count)
 {
     const int stride = gridDim.x * blockDim.x;
+    {
+        uint32_t *qn = ((t + 1) & 1) ? d.q1 : d.q0;
+        const int used = t == 0 ? 0 : (rel ? d.ctl[(rel - 1) * kCtlStride + kCtlCount] : prev_count);
+        for (int i = blockIdx.x * blockDim.x + (int)threadIdx.x; i < used; i += stride) qn[i] = kSentinel;
+    }
     for (int base = blockIdx.x * blockDim.x; base < d.n; base += stride) {
         const int j = base + (int)threadIdx.x;
         bool spk = false;
         if (j < d.n) spk = neuron_step(d, t, j, take_input(d, t, j), last != 0);
-        emit<true>(d, t, rel, base + ((int)threadIdx.x & ~(kWave - 1)), spk, j);
+        emit<true>(d, t, rel, base + ((int)threadIdx.x & ~(kWave - 1)), spk, j, d.n);
     }
     __syncthreads();
     int32_t *ctl = d.ctl + rel * kCtlStride;
@@ -293,8 +332,9 @@ __global__ void __launch_bounds__(256) k_ap(DevNet d, int t, int rel, int last)
             uint32_t spins = 0;
             for (;;) {
                 if (item < ld_agent(&ctl[kCtlCount])) { go = 1; break; }
-                if (__hip_atomic_load(&ctl[kCtlDone], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) ==
-                    (int)gridDim.x) {
+                if (ld_agent(&ctl[kCtlDone]) == (int)gridDim.x) {
+                    // every producer has finished: the count is final
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                     go = item < ld_agent(&ctl[kCtlCount]);
                     break;
                 }
@@ -310,7 +350,6 @@ __global__ void __launch_bounds__(256) k_ap(DevNet d, int t, int rel, int last)
                     if (++spins > kSpinLimit) { raise_error(d); go = 0; break; }
                     __builtin_amdgcn_s_sleep(1);
                 }
-                if (go) __hip_atomic_store(&q[item], kSentinel, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
         }
         go = __shfl(go, 0);
@@ -320,25 +359,350 @@ __global__ void __launch_bounds__(256) k_ap(DevNet d, int t, int rel, int last)
     }
 }
 
-// SKL scheme: the whole chunk in one co-resident launch; per step an
-// update phase, a grid barrier, a synapse-parallel scatter and a barrier.
-__global__ void __launch_bounds__(256) k_skl(DevNet d, int t0, int steps, int last_t)
+// ---------------------------------------------------------------------------
+// Slice-owned propagation (AB and SKL in atomic mode).  Workgroup p owns the
+// neurons of slice p.  Its input for step t is the part of every row queued
+// at step t-1 that targets slice p: a contiguous segment of the sorted row,
+// located by the offset table.  Segments are accumulated in LDS, so there
+// are no global float atomics and no pending-input array in HBM; the owner
+// then updates its neurons directly.
+// ---------------------------------------------------------------------------
+
+// Workgroups are dealt to the 8 XCDs round-robin; consecutive slices go to
+// the same XCD so rows' neighbouring segments and offsets share its L2.
+__device__ __forceinline__ int slice_of_block(int b, int g)
 {
+    return (g % 8 == 0) ? (b % 8) * (g / 8) + b / 8 : b;
+}
+
+// Bitmap and raster words of one wave covering [wave_base, wave_base+64)
+// of a slice kernel; returns the wave's spike mask.  Waves starting at or
+// beyond `limit` write nothing.
+__device__ __forceinline__ uint64_t emit_bits(const DevNet &d, int t, int rel, int wave_base, bool spk, int limit)
+{
+    const uint64_t mask = __ballot(spk);
+    if (wave_base >= limit) return mask;  // uniform over the wave
+    const int lane = lane_id();
+    uint32_t *bm = (t & 1) ? d.bm1 : d.bm0;
+    const int w = (wave_base >> 5) + lane;
+    if (lane < 2 && w < d.words) {
+        const uint32_t bits = lane ? (uint32_t)(mask >> 32) : (uint32_t)mask;
+        bm[w] = bits;
+        if (d.raster) d.raster[(size_t)rel * (size_t)d.words + (size_t)w] = bits;
+    }
+    return mask;
+}
+
+// Spike count of slice p at step t (waves' ballots of each tile) written to
+// bcnt[rel][p] for the host's per-step counts.  No global atomics.
+template <int R>
+__device__ __forceinline__ void slice_count(const DevNet &d, int rel, int p, int ntile, const uint64_t (&masks)[R],
+                                            int *s_cnt)
+{
+    if (threadIdx.x == 0) *s_cnt = 0;
+    __syncthreads();
+    int c = 0;
+#pragma unroll
+    for (int k = 0; k < R; ++k)
+        if (k < ntile) c += __popcll(masks[k]);
+    if (lane_id() == 0 && c) atomicAdd(s_cnt, c);
+    __syncthreads();
+    if (threadIdx.x == 0) d.bcnt[(size_t)rel * (size_t)d.P + (size_t)p] = *s_cnt;
+}
+
+// Exclusive prefix sum over the workgroup of one value per thread; returns
+// the thread's prefix and sets *total.  s_tmp holds blockDim/64 + 1 ints.
+__device__ __forceinline__ int block_exclusive_scan(int v, int *s_tmp, int *total)
+{
+    const int lane = lane_id();
+    const int wave = (int)threadIdx.x / kWave;
+    const int nw = (int)blockDim.x / kWave;
+    int incl = v;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const int y = __shfl_up(incl, off);
+        if (lane >= off) incl += y;
+    }
+    __syncthreads();
+    if (lane == kWave - 1) s_tmp[wave] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int w = 0; w < nw; ++w) {
+            const int c = s_tmp[w];
+            s_tmp[w] = run;
+            run += c;
+        }
+        s_tmp[nw] = run;
+    }
+    __syncthreads();
+    *total = s_tmp[nw];
+    return s_tmp[wave] + incl - v;
+}
+
+// LDS workspace of slice_accumulate beyond acc[L]: per spike of a pass its
+// row id, first segment index and the prefix of segment lengths, plus the
+// scan scratch.  A pass handles kSpikesPerThread spikes per thread.
+constexpr int kSpikesPerThread = 2;
+__host__ __device__ inline size_t slice_lds_bytes(int L, int block)
+{
+    const size_t cap = (size_t)kSpikesPerThread * (size_t)block;
+    return (size_t)L * 4 + cap * 4 /*ids*/ + cap * 4 /*seg start*/ + (cap + 1) * 4 /*seg prefix*/ +
+           ((size_t)block / 64 + 1) * 4;
+}
+
+// acc[0..len) <- input of slice p from the rows that spiked at step t-1.
+// Three dependent global levels, each with all its loads in flight:
+//  1. the step's bitmap, compacted in LDS into the ascending spike list
+//     (thread i owns a contiguous run of words);
+//  2. thread per spike: the row's segment bounds for slice p (offset table);
+//  3. thread per synapse entry of the concatenated segments (binary search
+//     over the LDS prefix of segment lengths).
+// Spikes beyond kSpikesPerThread*B are handled in further passes.  Ends with
+// a workgroup barrier.  LDS float atomics: any order.
+__device__ __forceinline__ void slice_accumulate(const DevNet &d, int t, int p, int lo, int len, float *acc,
+                                                 uint32_t *ws)
+{
+    const int B = (int)blockDim.x;
+    const int tid = (int)threadIdx.x;
+    const int cap = kSpikesPerThread * B;
+    uint32_t *ids = ws;
+    uint32_t *seg0 = ws + cap;
+    int *segp = (int *)(ws + 2 * cap);
+    int *scratch = (int *)(ws + 3 * cap + 1);
+    for (int i = tid; i < len; i += B) acc[i] = 0.0f;
+    if (t == 0) {
+        __syncthreads();
+        return;
+    }
+    const uint32_t *bm = ((t - 1) & 1) ? d.bm1 : d.bm0;
+    const int W = d.words;
+    const int per = (W + B - 1) / B;
+    const int wb = min(tid * per, W), we = min(wb + per, W);
+    // up to kKeep words stay in registers for the id pass below
+    constexpr int kKeep = 8;
+    uint32_t kept[kKeep];
+    int mine = 0;
+#pragma unroll
+    for (int k = 0; k < kKeep; ++k) {
+        kept[k] = (wb + k < we) ? bm[wb + k] : 0u;
+        mine += __popc(kept[k]);
+    }
+    for (int w = wb + kKeep; w < we; ++w) mine += __popc(bm[w]);
+    int total = 0;
+    const int base = block_exclusive_scan(mine, scratch, &total);
+    const size_t stride = (size_t)d.P + 1;
+    for (int pass = 0; pass < total; pass += cap) {
+        // 1. spike ids of this pass, ascending
+        if (mine > 0 && base < pass + cap && base + mine > pass) {
+            int idx = base;
+#pragma unroll
+            for (int k = 0; k < kKeep; ++k) {
+                uint32_t bits = kept[k];
+                while (bits) {
+                    if (idx >= pass && idx < pass + cap)
+                        ids[idx - pass] = (uint32_t)(wb + k) * 32u + (uint32_t)__builtin_ctz(bits);
+                    bits &= bits - 1u;
+                    ++idx;
+                }
+            }
+            for (int w = wb + kKeep; w < we && idx < pass + cap; ++w) {
+                uint32_t bits = bm[w];
+                while (bits) {
+                    if (idx >= pass && idx < pass + cap) ids[idx - pass] = (uint32_t)w * 32u + (uint32_t)__builtin_ctz(bits);
+                    bits &= bits - 1u;
+                    ++idx;
+                }
+            }
+        }
+        __syncthreads();
+        const int m = min(cap, total - pass);
+        // 2. segment bounds, thread per spike
+        int lens[kSpikesPerThread];
+        int lsum = 0;
+#pragma unroll
+        for (int k = 0; k < kSpikesPerThread; ++k) {
+            const int i = tid * kSpikesPerThread + k;
+            lens[k] = 0;
+            if (i < m) {
+                const uint16_t *o = d.soff + (size_t)ids[i] * stride + (size_t)p;
+                const uint32_t e0 = o[0], e1 = o[1];
+                seg0[i] = e0;
+                lens[k] = (int)(e1 - e0);
+            }
+            lsum += lens[k];
+        }
+        int E = 0;
+        int run = block_exclusive_scan(lsum, scratch, &E);
+#pragma unroll
+        for (int k = 0; k < kSpikesPerThread; ++k) {
+            const int i = tid * kSpikesPerThread + k;
+            if (i < m) segp[i] = run;
+            run += lens[k];
+        }
+        if (tid == 0) segp[m] = E;
+        __syncthreads();
+        // 3. entries, thread per synapse; four loads in flight per thread
+        for (int f0 = tid; f0 < E; f0 += 4 * B) {
+            uint2 x[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int f = f0 + k * B;
+                x[k] = make_uint2(0xFFFFFFFFu, 0u);
+                if (f < E) {
+                    int a = 0, b = m - 1;  // last spike i with segp[i] <= f
+                    while (a < b) {
+                        const int mid = (a + b + 1) >> 1;
+                        if (segp[mid] <= f) a = mid;
+                        else b = mid - 1;
+                    }
+                    x[k] = d.syn[(size_t)ids[a] * (size_t)d.nsyn + (size_t)seg0[a] + (size_t)(f - segp[a])];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (x[k].x != 0xFFFFFFFFu) atomicAdd(&acc[(int)x[k].x - lo], u2f(x[k].y));
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+}
+
+// Offset table: thread per (row i, boundary p): first index of row i whose
+// target is >= p*L (binary search in the ascending row).
+__global__ void __launch_bounds__(256) k_slice_offsets(DevNet d, uint16_t *soff)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t total = (int64_t)d.n * (d.P + 1);
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += stride) {
+        const int64_t i = g / (d.P + 1);
+        const int p = (int)(g - i * (d.P + 1));
+        const uint32_t key = (uint32_t)min<int64_t>((int64_t)p * d.L, d.n);
+        const uint2 *row = d.syn + (size_t)i * (size_t)d.nsyn;
+        int lo = 0, hi = d.nsyn;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (row[mid].x < key) lo = mid + 1;
+            else hi = mid;
+        }
+        soff[g] = (uint16_t)lo;
+    }
+}
+
+// ctl[rel].count = sum over slices of bcnt[rel][p]; one workgroup per step.
+__global__ void __launch_bounds__(256) k_reduce_counts(DevNet d, int steps)
+{
+    __shared__ int part[256];
+    for (int rel = blockIdx.x; rel < steps; rel += gridDim.x) {
+        int s = 0;
+        const int32_t *row = d.bcnt + (size_t)rel * (size_t)d.P;
+        for (int p = (int)threadIdx.x; p < d.P; p += blockDim.x) s += row[p];
+        part[threadIdx.x] = s;
+        __syncthreads();
+        for (int w = (int)blockDim.x / 2; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) d.ctl[rel * kCtlStride + kCtlCount] = part[0];
+        __syncthreads();
+    }
+}
+
+// AB scheme: one launch per step (rel-th of the chunk; t = tinfo[0] + rel),
+// grid = P slices.  Thread state is loaded from and stored to HBM.
+template <int R>
+__global__ void __launch_bounds__(1024) k_slice_step(DevNet d, int rel)
+{
+    extern __shared__ float dyn[];
+    __shared__ int s_cnt;
+    float *acc = dyn;
+    const int t = d.tinfo[0] + rel;
+    const bool last = t == d.tinfo[1];
+    const int p = slice_of_block((int)blockIdx.x, (int)gridDim.x);
+    const int lo = p * d.L;
+    const int len = min(d.L, d.n - lo);
+    slice_accumulate(d, t, p, lo, len, acc, (uint32_t *)(dyn + d.L));
+    const int B = (int)blockDim.x;
+    const int ntile = (len + B - 1) / B;
+    uint64_t masks[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        masks[k] = 0;
+        if (k >= ntile) continue;  // uniform
+        const int off = k * B + (int)threadIdx.x;
+        const int j = lo + off;
+        bool spk = false;
+        if (off < len) spk = neuron_step(d, t, j, acc[off], last);
+        masks[k] = emit_bits(d, t, rel, lo + k * B + ((int)threadIdx.x & ~(kWave - 1)), spk, lo + len);
+    }
+    slice_count<R>(d, rel, p, ntile, masks, &s_cnt);
+}
+
+// SKL scheme: the whole chunk in one co-resident launch of P workgroups.
+// Each thread keeps the state of its R neurons (j = lo + k*B + tid) in
+// registers for the whole chunk; per step: slice accumulation from the
+// previous step's region queue, update, spike output, one grid barrier.
+template <int R>
+__global__ void __launch_bounds__(1024) k_skl_slice(DevNet d, int t0, int steps, int last_t)
+{
+    extern __shared__ float dyn[];
+    __shared__ int s_cnt;
+    float *acc = dyn;
+    uint32_t *ws = (uint32_t *)(dyn + d.L);
+    const int p = slice_of_block((int)blockIdx.x, (int)gridDim.x);
+    const int lo = p * d.L;
+    const int len = min(d.L, d.n - lo);
+    const int B = (int)blockDim.x;
+    const int tid = (int)threadIdx.x;
+    const int ntile = (len + B - 1) / B;
+    float v[R], u[R], a[R], b[R], c[R], dd[R], g[R];
+    int nos[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int off = k * B + tid;
+        nos[k] = 0;
+        v[k] = u[k] = a[k] = b[k] = c[k] = dd[k] = g[k] = 0.0f;
+        if (off < len) {
+            const int j = lo + off;
+            v[k] = d.v[j]; u[k] = d.u[j];
+            a[k] = d.a[j]; b[k] = d.b[j]; c[k] = d.c[j]; dd[k] = d.d[j];
+            g[k] = (j < d.n_exc) ? d.ge : d.gi;
+        }
+    }
     uint32_t epoch = 0;
-    const int stride = gridDim.x * blockDim.x;
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     for (int rel = 0; rel < steps; ++rel) {
         const int t = t0 + rel;
-        for (int base = blockIdx.x * blockDim.x; base < d.n; base += stride) {
-            const int j = base + (int)threadIdx.x;
+        slice_accumulate(d, t, p, lo, len, acc, ws);
+        uint64_t masks[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            masks[k] = 0;
+            if (k >= ntile) continue;  // uniform
+            const int off = k * B + tid;
+            const int j = lo + off;
             bool spk = false;
-            if (j < d.n) spk = neuron_step(d, t, j, take_input(d, t, j), t == last_t);
-            emit<false>(d, t, rel, base + ((int)threadIdx.x & ~(kWave - 1)), spk, j);
+            if (off < len) {
+                const float z = thalamic(d.seed, (uint32_t)t, (uint32_t)j);
+                float I = g[k] * z;
+                I = I + acc[off];
+                spk = izh_step(v[k], u[k], a[k], b[k], c[k], dd[k], I);
+                nos[k] += spk ? 1 : 0;
+                if (t == last_t) d.lastI[j] = I;
+            }
+            masks[k] = emit_bits(d, t, rel, lo + k * B + (tid & ~(kWave - 1)), spk, lo + len);
         }
-        if (!grid_sync(d, epoch)) return;
-        const int count = d.ctl[rel * kCtlStride + kCtlCount];
-        push_flat(d, t, count, gid, (int64_t)stride);
-        if (!grid_sync(d, epoch)) return;
+        slice_count<R>(d, rel, p, ntile, masks, &s_cnt);
+        if (!grid_sync(d, epoch)) break;
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int off = k * B + tid;
+        if (off < len) {
+            const int j = lo + off;
+            d.v[j] = v[k];
+            d.u[j] = u[k];
+            d.nospks[j] += nos[k];
+        }
     }
 }
 
@@ -351,7 +715,7 @@ __global__ void __launch_bounds__(256) k_pull_thread(DevNet d, int t, int rel, i
         const int j = base + (int)threadIdx.x;
         bool spk = false;
         if (j < d.n) spk = neuron_step(d, t, j, gather_input(d, t, j), last != 0);
-        emit<false>(d, t, rel, base + ((int)threadIdx.x & ~(kWave - 1)), spk, j);
+        emit<false>(d, t, rel, base + ((int)threadIdx.x & ~(kWave - 1)), spk, j, d.n);
     }
 }
 
@@ -393,7 +757,7 @@ __global__ void __launch_bounds__(256) k_pull_wave(DevNet d, int t, int rel, int
         const int j = wb + lane;
         bool spk = false;
         if (j < d.n) spk = neuron_step(d, t, j, mine, last != 0);
-        emit<false>(d, t, rel, wb, spk, j);
+        emit<false>(d, t, rel, wb, spk, j, d.n);
     }
 }
 
@@ -412,7 +776,7 @@ __global__ void __launch_bounds__(256) k_ap_det(DevNet d, int t, int rel, int la
         const int j = tile * kWave + lane;
         bool spk = false;
         if (j < d.n) spk = neuron_step(d, t, j, gather_input(d, t, j), last != 0);
-        emit<false>(d, t, rel, tile * kWave, spk, j);
+        emit<false>(d, t, rel, tile * kWave, spk, j, d.n);
     }
 }
 
@@ -428,22 +792,9 @@ __global__ void __launch_bounds__(256) k_skl_det(DevNet d, int t0, int steps, in
             const int j = base + (int)threadIdx.x;
             bool spk = false;
             if (j < d.n) spk = neuron_step(d, t, j, gather_input(d, t, j), t == last_t);
-            emit<false>(d, t, rel, base + ((int)threadIdx.x & ~(kWave - 1)), spk, j);
+            emit<false>(d, t, rel, base + ((int)threadIdx.x & ~(kWave - 1)), spk, j, d.n);
         }
         if (!grid_sync(d, epoch)) return;
-    }
-}
-
-// Switching from det to atomic mode at step t: the pending input of step t
-// is gathered from step t-1's bitmap into isyn[t&1]; isyn[(t+1)&1] cleared.
-__global__ void __launch_bounds__(256) k_gather_isyn(DevNet d, int t)
-{
-    float *in = (t & 1) ? d.isyn1 : d.isyn0;
-    float *nx = (t & 1) ? d.isyn0 : d.isyn1;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < d.n; j += stride) {
-        in[j] = gather_input(d, t, (int)j);
-        nx[j] = 0.0f;
     }
 }
 
@@ -470,9 +821,33 @@ int grid_for(int64_t n, int block)
     return (int)g;
 }
 
-hipError_t coop(const void *fn, const Launch &L, void **args)
+hipError_t coop(const void *fn, const Launch &L, void **args, size_t shmem = 0)
 {
-    return hipLaunchCooperativeKernel(fn, dim3(L.grid), dim3(L.block), args, 0, L.stream);
+    return hipLaunchCooperativeKernel(fn, dim3(L.grid), dim3(L.block), args, shmem, L.stream);
+}
+
+template <int R>
+hipError_t launch_skl_slice_r(const DevNet &d, int t0, int steps, int last_t, const Launch &L)
+{
+    DevNet dd = d;
+    void *args[] = {&dd, &t0, &steps, &last_t};
+    return coop((const void *)k_skl_slice<R>, L, args, slice_lds_bytes(d.L, L.block));
+}
+
+template <int R>
+hipError_t launch_slice_step_r(const DevNet &d, int rel, const Launch &L)
+{
+    hipLaunchKernelGGL(k_slice_step<R>, dim3(L.grid), dim3(L.block), slice_lds_bytes(d.L, L.block), L.stream, d, rel);
+    return hipGetLastError();
+}
+
+const void *skl_slice_fn(int per_thread)
+{
+    if (per_thread <= 1) return (const void *)k_skl_slice<1>;
+    if (per_thread <= 2) return (const void *)k_skl_slice<2>;
+    if (per_thread <= 4) return (const void *)k_skl_slice<4>;
+    if (per_thread <= 8) return (const void *)k_skl_slice<8>;
+    return (const void *)k_skl_slice<16>;
 }
 
 }  // namespace
@@ -503,29 +878,69 @@ hipError_t launch_push_rows(const DevNet &d, int t, int rel, const Launch &L)
     return hipGetLastError();
 }
 
-hipError_t launch_push_flat(const DevNet &d, int t, int rel, const Launch &L, bool cooperative)
+hipError_t launch_push_bitmap(const DevNet &d, int t, hipStream_t s)
 {
-    if (!cooperative) {
-        hipLaunchKernelGGL(k_push_flat, dim3(L.grid), dim3(L.block), 0, L.stream, d, t, rel);
-        return hipGetLastError();
-    }
-    DevNet dd = d;
-    void *args[] = {&dd, &t, &rel};
-    return coop((const void *)k_push_flat, L, args);
+    const int64_t groups = ((int64_t)d.words + 1) / 2;
+    hipLaunchKernelGGL(k_push_bitmap, dim3(grid_for(groups * 64, 256)), dim3(256), 0, s, d, t);
+    return hipGetLastError();
 }
 
-hipError_t launch_ap(const DevNet &d, int t, int rel, int last, const Launch &L)
+hipError_t launch_reduce_counts(const DevNet &d, int steps, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_reduce_counts, dim3(grid_for(steps, 256)), dim3(256), 0, s, d, steps);
+    return hipGetLastError();
+}
+
+hipError_t launch_push_flat(const DevNet &d, int t, int rel, const Launch &L)
+{
+    hipLaunchKernelGGL(k_push_flat, dim3(L.grid), dim3(L.block), 0, L.stream, d, t, rel);
+    return hipGetLastError();
+}
+
+hipError_t launch_ap(const DevNet &d, int t, int rel, int last, int prev_count, const Launch &L)
 {
     DevNet dd = d;
-    void *args[] = {&dd, &t, &rel, &last};
+    void *args[] = {&dd, &t, &rel, &last, &prev_count};
     return coop((const void *)k_ap, L, args);
 }
 
-hipError_t launch_skl(const DevNet &d, int t0, int steps, int last_t, const Launch &L)
+hipError_t launch_slice_offsets(const DevNet &d, uint16_t *soff, hipStream_t s)
 {
-    DevNet dd = d;
-    void *args[] = {&dd, &t0, &steps, &last_t};
-    return coop((const void *)k_skl, L, args);
+    const int64_t total = (int64_t)d.n * (d.P + 1);
+    hipLaunchKernelGGL(k_slice_offsets, dim3(grid_for(total, 256)), dim3(256), 0, s, d, soff);
+    return hipGetLastError();
+}
+
+hipError_t launch_slice_step(const DevNet &d, int rel, const Launch &L)
+{
+    const int per_thread = (d.L + L.block - 1) / L.block;
+    if (per_thread <= 1) return launch_slice_step_r<1>(d, rel, L);
+    if (per_thread <= 2) return launch_slice_step_r<2>(d, rel, L);
+    if (per_thread <= 4) return launch_slice_step_r<4>(d, rel, L);
+    if (per_thread <= 8) return launch_slice_step_r<8>(d, rel, L);
+    if (per_thread <= 16) return launch_slice_step_r<16>(d, rel, L);
+    return hipErrorInvalidValue;
+}
+
+
+hipError_t launch_skl_slice(const DevNet &d, int t0, int steps, int last_t, const Launch &L)
+{
+    const int per_thread = (d.L + L.block - 1) / L.block;
+    if (per_thread <= 1) return launch_skl_slice_r<1>(d, t0, steps, last_t, L);
+    if (per_thread <= 2) return launch_skl_slice_r<2>(d, t0, steps, last_t, L);
+    if (per_thread <= 4) return launch_skl_slice_r<4>(d, t0, steps, last_t, L);
+    if (per_thread <= 8) return launch_skl_slice_r<8>(d, t0, steps, last_t, L);
+    if (per_thread <= 16) return launch_skl_slice_r<16>(d, t0, steps, last_t, L);
+    return hipErrorInvalidValue;
+}
+
+int skl_slice_blocks_per_cu(int L, int block)
+{
+    int per_cu = 0;
+    const void *fn = skl_slice_fn((L + block - 1) / block);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, slice_lds_bytes(L, block)) != hipSuccess)
+        return 0;
+    return per_cu;
 }
 
 hipError_t launch_pull_thread(const DevNet &d, int t, int rel, int last, const Launch &L, bool cooperative)
@@ -559,12 +974,6 @@ hipError_t launch_skl_det(const DevNet &d, int t0, int steps, int last_t, const 
     return coop((const void *)k_skl_det, L, args);
 }
 
-hipError_t launch_gather_isyn(const DevNet &d, int t, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_gather_isyn, dim3(grid_for(d.n, 256)), dim3(256), 0, s, d, t);
-    return hipGetLastError();
-}
-
 hipError_t launch_thalamic(uint64_t seed, uint32_t t, int64_t n, float *out, hipStream_t s)
 {
     hipLaunchKernelGGL(k_thalamic, dim3(grid_for(n, 256)), dim3(256), 0, s, seed, t, n, out);
@@ -577,22 +986,27 @@ hipError_t launch_normal_words(const uint32_t *x0, const uint32_t *x1, int64_t n
     return hipGetLastError();
 }
 
-int coop_max_blocks(CoopKernel k, int block)
+int coop_blocks_per_cu(CoopKernel k, int block, size_t shmem)
 {
     const void *fn = nullptr;
     switch (k) {
-    case kCoopPushFlat: fn = (const void *)k_push_flat; break;
     case kCoopAP: fn = (const void *)k_ap; break;
-    case kCoopSKL: fn = (const void *)k_skl; break;
+    case kCoopSKLSlice: fn = (const void *)k_skl_slice<16>; break;
     case kCoopPull: fn = (const void *)k_pull_thread; break;
     case kCoopAPDet: fn = (const void *)k_ap_det; break;
     case kCoopSKLDet: fn = (const void *)k_skl_det; break;
     }
-    int per_cu = 0, dev = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, 0) != hipSuccess) return 0;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, shmem) != hipSuccess) return 0;
+    return per_cu;
+}
+
+int device_cus()
+{
+    int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-    return per_cu * cus;
+    return cus;
 }
 
 }  // namespace g4s

@@ -58,8 +58,21 @@ struct g4s_net {
     uint32_t *h_raster = nullptr;  // pinned staging
     int32_t *h_ctl = nullptr;      // pinned staging
     bool det_ready = false;
-    int last_accum = G4S_ACCUM_ATOMIC;  // mode of the previous step
+    bool slices_ready = false;
+    // Per-step AB launches captured once per chunk length and device layout.
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_steps = 0, graph_block = 0;
+    int64_t gen = 0, graph_gen = -1;  // gen: bumped when DevNet changes
+    int32_t *h_tinfo = nullptr;       // pinned staging of d.tinfo
+    // isyn0/isyn1 hold the pending input only after the global-atomic schemes
+    // (N, S, AP atomic); the slice and gather schemes derive the input from
+    // the previous step's queue or bitmap instead.
+    bool isyn_valid = true;
+    int32_t last_count = 0;     // spikes of step t-1
     int64_t est_count = 0;      // recent max spikes per step (grid sizing)
+    int coop_cache[8][5] = {};  // co-resident blocks per CU per (kernel, block/64)
+    int cus = 0;
     g4s_timing timing{};
     std::vector<void *> allocs;
 
@@ -86,9 +99,16 @@ int ensure_chunk(g4s_net *net, int chunk)
     int rc;
     if ((rc = net->alloc(&net->d.raster, (size_t)chunk * net->d.words))) return rc;
     if ((rc = net->alloc(&net->d.ctl, (size_t)chunk * kCtlStride))) return rc;
+    if (net->d.bcnt) {
+        hipFree(net->d.bcnt);
+        net->allocs.erase(std::find(net->allocs.begin(), net->allocs.end(), (void *)net->d.bcnt));
+        net->d.bcnt = nullptr;
+    }
+    if (net->d.P > 0 && (rc = net->alloc(&net->d.bcnt, (size_t)chunk * net->d.P))) return rc;
     HIP_OK(hipHostMalloc((void **)&net->h_raster, (size_t)chunk * net->d.words * sizeof(uint32_t)));
     HIP_OK(hipHostMalloc((void **)&net->h_ctl, (size_t)chunk * kCtlStride * sizeof(int32_t)));
     net->chunk_cap = chunk;
+    net->gen += 1;
     return G4S_OK;
 }
 
@@ -111,6 +131,7 @@ int build_det_tables(g4s_net *net)
     net->d.in_ptr = dp;
     net->d.in_syn = dt;
     net->det_ready = true;
+    net->gen += 1;
     return G4S_OK;
 }
 
@@ -135,13 +156,115 @@ Launch plain(g4s_net *net, int64_t threads, int block)
     return Launch{(int)g, block, net->stream};
 }
 
+// Grid for a cooperative kernel.  Kernels whose workgroups wait on each other
+// use at most half the co-resident capacity the occupancy API reports: the
+// API can overstate it (it does not see the SGPR limit of gfx9-family SIMDs,
+// 800 per SIMD, which caps kernels above 96 SGPRs at 7 waves), and a grid
+// that is not fully resident would wait forever.
 int coop_launch(g4s_net *net, CoopKernel k, int block, int64_t useful_threads, Launch *L)
 {
-    int maxb = coop_max_blocks(k, block);
-    if (maxb <= 0) return fail(G4S_ERR_HIP, "no co-resident capacity for cooperative kernel %d", (int)k);
+    int &cached = net->coop_cache[(int)k][block / 64];
+    if (cached == 0) cached = coop_blocks_per_cu(k, block, 0);
+    if (net->cus == 0) net->cus = device_cus();
+    if (cached <= 0 || net->cus <= 0) return fail(G4S_ERR_HIP, "no co-resident capacity for cooperative kernel %d", (int)k);
+    const int per_cu = std::max(1, cached / 2);
+    const int maxb = per_cu * net->cus;
     int64_t want = (useful_threads + block - 1) / block;
     int g = (int)std::max<int64_t>(1, std::min<int64_t>(maxb, want));
     *L = Launch{g, block, net->stream};
+    return G4S_OK;
+}
+
+// Slice partition for the LDS-accumulating schemes: `want` slices (0 = two
+// per CU), slice length L a multiple of 64 and at most 16*1024 neurons;
+// offsets stored as uint16 (Nsyn < 65536).  Rebuilt when the slice count
+// changes.
+int64_t pick_slice_len(int64_t n, int64_t want)
+{
+    int64_t P = std::max<int64_t>(1, std::min<int64_t>(want, (n + 63) / 64));
+    int64_t L = ((n + P - 1) / P + 63) / 64 * 64;
+    return std::min<int64_t>(L, 8 * 1024);
+}
+
+int build_slices(g4s_net *net, int want)
+{
+    if (net->d.nsyn > 65535) return fail(G4S_ERR_UNSUPPORTED, "slice schemes need Nsyn < 65536");
+    if (net->cus == 0) net->cus = device_cus();
+    const int64_t n = net->d.n;
+    if (want <= 0) want = 2 * std::max(net->cus, 1);
+    const int64_t L = pick_slice_len(n, want);
+    const int64_t P = (n + L - 1) / L;
+    if (net->slices_ready && net->d.L == L) return G4S_OK;
+    // LDS holds acc[L] plus the pass workspace (at most 64 KB per workgroup)
+    if (L * 4 > 32 * 1024)
+        return fail(G4S_ERR_UNSUPPORTED, "N=%lld needs %lld slices of %lld: beyond the slice schemes' LDS", (long long)n,
+                    (long long)P, (long long)L);
+    for (void **pp : {(void **)&net->d.soff, (void **)&net->d.bcnt}) {
+        if (*pp) {
+            hipFree(*pp);
+            net->allocs.erase(std::find(net->allocs.begin(), net->allocs.end(), *pp));
+            *pp = nullptr;
+        }
+    }
+    net->d.P = (int32_t)P;
+    net->d.L = (int32_t)L;
+    uint16_t *soff = nullptr;
+    int rc;
+    if ((rc = net->alloc(&soff, (size_t)n * (size_t)(P + 1)))) return rc;
+    if ((rc = net->alloc(&net->d.bcnt, (size_t)std::max(net->chunk_cap, 1) * (size_t)P))) return rc;
+    HIP_OK(launch_slice_offsets(net->d, soff, net->stream));
+    HIP_OK(hipStreamSynchronize(net->stream));
+    net->d.soff = soff;
+    net->slices_ready = true;
+    net->gen += 1;
+    return G4S_OK;
+}
+
+// Workgroup size of the slice kernels: requested, or one neuron per thread
+// up to 1024 threads.
+int slice_block(const g4s_net *net, int requested)
+{
+    if (requested > 0) return requested;
+    return (int)std::min<int64_t>(1024, std::max<int64_t>(64, net->d.L));
+}
+
+void drop_graph(g4s_net *net)
+{
+    if (net->graph_exec) hipGraphExecDestroy(net->graph_exec);
+    if (net->graph) hipGraphDestroy(net->graph);
+    net->graph_exec = nullptr;
+    net->graph = nullptr;
+    net->graph_steps = 0;
+}
+
+// AB chunk of `cs` steps: replayed from a captured HIP graph of cs
+// launches (captured again when the chunk length or any device pointer of
+// the network changed), or launched directly.
+int run_ab_chunk(g4s_net *net, int cs, bool use_graph, int block)
+{
+    const Launch L{net->d.P, block, net->stream};
+    if (!use_graph) {
+        for (int rel = 0; rel < cs; ++rel) HIP_OK(launch_slice_step(net->d, rel, L));
+        return G4S_OK;
+    }
+    if (!net->graph_exec || net->graph_steps != cs || net->graph_gen != net->gen || net->graph_block != block) {
+        drop_graph(net);
+        HIP_OK(hipStreamBeginCapture(net->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e = hipSuccess;
+        for (int rel = 0; rel < cs && e == hipSuccess; ++rel) e = launch_slice_step(net->d, rel, L);
+        hipGraph_t g = nullptr;
+        hipError_t e2 = hipStreamEndCapture(net->stream, &g);
+        if (e != hipSuccess || e2 != hipSuccess) {
+            if (g) hipGraphDestroy(g);
+            return fail(G4S_ERR_HIP, "graph capture of %d steps: %s", cs, hipGetErrorString(e != hipSuccess ? e : e2));
+        }
+        net->graph = g;
+        HIP_OK(hipGraphInstantiate(&net->graph_exec, g, nullptr, nullptr, 0));
+        net->graph_steps = cs;
+        net->graph_gen = net->gen;
+        net->graph_block = block;
+    }
+    HIP_OK(hipGraphLaunch(net->graph_exec, net->stream));
     return G4S_OK;
 }
 
@@ -203,6 +326,11 @@ int g4s_create(const g4s_spec *spec, g4s_net **out)
     if ((rc = net->alloc(&d.q1, n))) return bail(rc);
     if ((rc = net->alloc(&d.bar, 1))) return bail(rc);
     if ((rc = net->alloc(&d.err, 1))) return bail(rc);
+    int32_t *tinfo = nullptr;
+    if ((rc = net->alloc(&tinfo, 2))) return bail(rc);
+    d.tinfo = tinfo;
+    if (hipHostMalloc((void **)&net->h_tinfo, 2 * sizeof(int32_t)) != hipSuccess)
+        return bail(fail(G4S_ERR_MEM, "hipHostMalloc failed"));
     uint2 *syn = nullptr;
     const size_t ns = n * (size_t)spec->nsyn;
     if ((rc = net->alloc(&syn, ns))) return bail(rc);
@@ -226,12 +354,61 @@ int g4s_reset(g4s_net *net)
     HIP_OK(launch_fill_u32((uint32_t *)net->d.err, 1, 0u, net->stream));
     HIP_OK(hipStreamSynchronize(net->stream));
     net->t = 0;
-    net->last_accum = G4S_ACCUM_ATOMIC;
+    net->isyn_valid = true;
+    net->last_count = 0;
     net->est_count = std::max<int64_t>(64, net->spec.n / 16);
     return G4S_OK;
 }
 
 int64_t g4s_current_step(const g4s_net *net) { return net ? net->t : -1; }
+
+namespace {
+
+bool uses_isyn(int alg, int accum)
+{
+    return accum == G4S_ACCUM_ATOMIC && (alg == G4S_ALG_AP || alg == G4S_ALG_N || alg == G4S_ALG_S);
+}
+
+// Launches of one step of a per-step scheme; returns the HIP status.
+hipError_t launch_step(g4s_net *net, int alg, int accum, int block, int t, int rel, int last, int *launches,
+                       Launch *L, int *rc)
+{
+    DevNet &d = net->d;
+    *rc = G4S_OK;
+    if (accum == G4S_ACCUM_DET) {
+        *launches += 1;
+        switch (alg) {
+        case G4S_ALG_AP:
+            if ((*rc = coop_launch(net, kCoopAPDet, block, d.n, L))) return hipSuccess;
+            return launch_ap_det(d, t, rel, last, *L);
+        case G4S_ALG_N:
+            *L = plain(net, d.n, block);
+            return launch_pull_thread(d, t, rel, last, *L, false);
+        case G4S_ALG_S:
+            *L = plain(net, ((int64_t)d.n + 63) / 64 * 64, block);
+            return launch_pull_wave(d, t, rel, last, *L);
+        default:  // AB
+            if ((*rc = coop_launch(net, kCoopPull, block, d.n, L))) return hipSuccess;
+            return launch_pull_thread(d, t, rel, last, *L, true);
+        }
+    }
+    if (alg == G4S_ALG_AP) {
+        *launches += 1;
+        if ((*rc = coop_launch(net, kCoopAP, block, d.n, L))) return hipSuccess;
+        return launch_ap(d, t, rel, last, net->last_count, *L);
+    }
+    *launches += 2;
+    hipError_t e = launch_update(d, t, rel, last, plain(net, d.n, block));
+    if (e != hipSuccess) return e;
+    if (alg == G4S_ALG_N) {
+        *L = plain(net, net->est_count * 64, block);
+        return launch_push_rows(d, t, rel, *L);
+    }
+    *L = plain(net, net->est_count * (int64_t)d.nsyn, block);
+    return launch_push_flat(d, t, rel, *L);
+}
+
+}  // namespace
 
 int g4s_run(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, int32_t *counts,
             const g4s_run_opts *opts)
@@ -243,25 +420,49 @@ int g4s_run(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, i
     if (net->t + (int64_t)steps > 0x7FFFFFFFLL) return fail(G4S_ERR_UNSUPPORTED, "step index overflow");
     g4s_run_opts o{};
     if (opts) o = *opts;
-    const int block = o.block > 0 ? o.block : 256;
-    if (block % 64 != 0 || block > 256) return fail(G4S_ERR_ARG, "block must be a multiple of 64 <= 256");
+    if (o.block < 0 || o.block % 64 != 0 || o.block > 1024) return fail(G4S_ERR_ARG, "block must be a multiple of 64 <= 1024");
+    const int block = o.block > 0 ? std::min(o.block, 256) : 256;  // launch bound of the other kernels
     const int chunk = o.chunk > 0 ? o.chunk : 1000;
     int rc;
     if ((rc = ensure_chunk(net, chunk))) return rc;
+    const bool slice = accum == G4S_ACCUM_ATOMIC && (alg == G4S_ALG_AB || alg == G4S_ALG_SKL);
     if (accum == G4S_ACCUM_DET && (rc = build_det_tables(net))) return rc;
+    if (slice && (rc = build_slices(net, o.slices))) return rc;
+    const int sblock = slice ? slice_block(net, o.block) : 0;
+    if (slice && (int64_t)net->d.L > 16 * (int64_t)sblock)
+        return fail(G4S_ERR_ARG, "block %d too small for slices of %d neurons (max 16 per thread)", sblock, net->d.L);
 
     DevNet &d = net->d;
+    Launch L{};
+    if (alg == G4S_ALG_SKL && slice) {
+        // every workgroup must be resident: P slices within half the
+        // capacity reported for the kernel instantiation that will run
+        const int per_cu = skl_slice_blocks_per_cu(d.L, sblock);
+        const int cap = std::max(1, per_cu / 2) * std::max(net->cus, 1);
+        if (per_cu <= 0 || cap < d.P)
+            return fail(G4S_ERR_UNSUPPORTED, "SKL needs %d co-resident workgroups, device holds %d", d.P, cap);
+        L = Launch{d.P, sblock, net->stream};
+    }
+    // The global-atomic schemes read the pending input from isyn[t&1]; after
+    // a scheme that keeps none, rebuild it from the queue of step t-1.  This
+    // must precede the AP sentinel fill below, which overwrites that queue.
+    if (uses_isyn(alg, accum) && !net->isyn_valid && steps > 0) {
+        const int t = (int)net->t;
+        float *cur = (t & 1) ? d.isyn1 : d.isyn0;
+        float *nxt = (t & 1) ? d.isyn0 : d.isyn1;
+        HIP_OK(launch_fill_u32((uint32_t *)cur, (size_t)d.n, 0u, net->stream));
+        HIP_OK(launch_fill_u32((uint32_t *)nxt, (size_t)d.n, 0u, net->stream));
+        if (t > 0) HIP_OK(launch_push_bitmap(d, t - 1, net->stream));
+    }
+    if (steps > 0) net->isyn_valid = uses_isyn(alg, accum);
     // AP consumers wait on kSentinel slots: restore the invariant when AP
     // follows another scheme (those store queue slots without clearing).
-    if (alg == G4S_ALG_AP && accum == G4S_ACCUM_ATOMIC) {
+    if (alg == G4S_ALG_AP && accum == G4S_ACCUM_ATOMIC && steps > 0) {
         HIP_OK(launch_fill_u32(d.q0, (size_t)d.n, kSentinel, net->stream));
         HIP_OK(launch_fill_u32(d.q1, (size_t)d.n, kSentinel, net->stream));
     }
-    // det mode keeps no pending-input buffers; rebuild them when switching
-    // back to atomic mode in the middle of a simulation.
-    if (accum == G4S_ACCUM_ATOMIC && net->last_accum == G4S_ACCUM_DET && net->t > 0)
-        HIP_OK(launch_gather_isyn(d, (int)net->t, net->stream));
-    if (steps > 0) net->last_accum = accum;
+    const bool use_graph = o.graph >= 0;
+
     const auto wall0 = std::chrono::steady_clock::now();
     double dev_ms = 0.0;
     int64_t spikes = 0;
@@ -274,68 +475,38 @@ int g4s_run(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, i
         const int t0 = (int)(net->t);
         HIP_OK(launch_fill_u32((uint32_t *)d.ctl, (size_t)cs * kCtlStride, 0u, net->stream));
         HIP_OK(launch_fill_u32(d.bar, 1, 0u, net->stream));
+        if (alg == G4S_ALG_AB && slice) {
+            net->h_tinfo[0] = t0;
+            net->h_tinfo[1] = (int32_t)(t_end - 1);
+            HIP_OK(hipMemcpyAsync((void *)d.tinfo, net->h_tinfo, 2 * sizeof(int32_t), hipMemcpyHostToDevice, net->stream));
+        }
         HIP_OK(hipEventRecord(net->ev0, net->stream));
-        Launch L{};
-        if (alg == G4S_ALG_SKL) {
-            const CoopKernel k = accum == G4S_ACCUM_DET ? kCoopSKLDet : kCoopSKL;
-            if ((rc = coop_launch(net, k, block, std::max<int64_t>(d.n, (int64_t)net->est_count * d.nsyn), &L))) return rc;
-            hipError_t e = accum == G4S_ACCUM_DET ? launch_skl_det(d, t0, cs, (int)(t_end - 1), L)
-                                                  : launch_skl(d, t0, cs, (int)(t_end - 1), L);
+        if (alg == G4S_ALG_AB && slice) {
+            if ((rc = run_ab_chunk(net, cs, use_graph, sblock))) return rc;
+            launches += cs;
+            dom_grid = d.P;
+        } else if (alg == G4S_ALG_SKL) {
+            hipError_t e;
+            if (slice) {
+                e = launch_skl_slice(d, t0, cs, (int)(t_end - 1), L);
+            } else {
+                if ((rc = coop_launch(net, kCoopSKLDet, block, d.n, &L))) return rc;
+                e = launch_skl_det(d, t0, cs, (int)(t_end - 1), L);
+            }
             if (e != hipSuccess) return fail(G4S_ERR_HIP, "SKL cooperative launch (grid %d): %s", L.grid, hipGetErrorString(e));
             launches += 1;
             dom_grid = L.grid;
         } else {
             for (int rel = 0; rel < cs; ++rel) {
                 const int t = t0 + rel;
-                const int last = (t == t_end - 1);
-                hipError_t e = hipSuccess;
-                if (accum == G4S_ACCUM_DET) {
-                    switch (alg) {
-                    case G4S_ALG_AP:
-                        if ((rc = coop_launch(net, kCoopAPDet, block, d.n, &L))) return rc;
-                        e = launch_ap_det(d, t, rel, last, L);
-                        break;
-                    case G4S_ALG_N:
-                        L = plain(net, d.n, block);
-                        e = launch_pull_thread(d, t, rel, last, L, false);
-                        break;
-                    case G4S_ALG_S:
-                        L = plain(net, ((int64_t)d.n + 63) / 64 * 64, block);
-                        e = launch_pull_wave(d, t, rel, last, L);
-                        break;
-                    case G4S_ALG_AB:
-                        if ((rc = coop_launch(net, kCoopPull, block, d.n, &L))) return rc;
-                        e = launch_pull_thread(d, t, rel, last, L, true);
-                        break;
-                    }
-                    launches += 1;
-                } else {
-                    if (alg == G4S_ALG_AP) {
-                        if ((rc = coop_launch(net, kCoopAP, block, d.n, &L))) return rc;
-                        e = launch_ap(d, t, rel, last, L);
-                        launches += 1;
-                    } else {
-                        e = launch_update(d, t, rel, last, plain(net, d.n, block));
-                        if (e == hipSuccess) {
-                            if (alg == G4S_ALG_N) {
-                                L = plain(net, net->est_count * 64, block);
-                                e = launch_push_rows(d, t, rel, L);
-                            } else if (alg == G4S_ALG_S) {
-                                L = plain(net, net->est_count * (int64_t)d.nsyn, block);
-                                e = launch_push_flat(d, t, rel, L, false);
-                            } else {
-                                if ((rc = coop_launch(net, kCoopPushFlat, block, (int64_t)d.n * d.nsyn, &L))) return rc;
-                                e = launch_push_flat(d, t, rel, L, true);
-                            }
-                        }
-                        launches += 2;
-                    }
-                }
+                hipError_t e = launch_step(net, alg, accum, block, t, rel, t == t_end - 1, &launches, &L, &rc);
+                if (rc) return rc;
                 if (e != hipSuccess) return fail(G4S_ERR_HIP, "step %d launch (alg %d, grid %d): %s", t, alg, L.grid, hipGetErrorString(e));
                 dom_grid = L.grid;
             }
         }
         HIP_OK(hipEventRecord(net->ev1, net->stream));
+        if (slice) HIP_OK(launch_reduce_counts(d, cs, net->stream));
         HIP_OK(hipMemcpyAsync(net->h_ctl, d.ctl, (size_t)cs * kCtlStride * sizeof(int32_t), hipMemcpyDeviceToHost, net->stream));
         if (raster)
             HIP_OK(hipMemcpyAsync(net->h_raster, d.raster, (size_t)cs * words * sizeof(uint32_t), hipMemcpyDeviceToHost, net->stream));
@@ -353,6 +524,7 @@ int g4s_run(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, i
             mx = std::max<int64_t>(mx, cnt);
             if (counts) counts[c0 + r] = cnt;
         }
+        net->last_count = net->h_ctl[(cs - 1) * kCtlStride + kCtlCount];
         net->est_count = std::max<int64_t>(64, mx + mx / 4);
         if (raster) memcpy(raster + (size_t)c0 * words, net->h_raster, (size_t)cs * words * sizeof(uint32_t));
         net->t += cs;
@@ -363,6 +535,20 @@ int g4s_run(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, i
     net->timing.spikes = spikes;
     net->timing.launches = launches;
     net->timing.grid = dom_grid;
+    return G4S_OK;
+}
+
+int g4s_set_regime(g4s_net *net, int regime, float thal_scale)
+{
+    if (!net) return fail(G4S_ERR_ARG, "net is NULL");
+    g4s_spec s = net->spec;
+    s.regime = regime;
+    s.thal_scale = thal_scale;
+    int rc = validate_spec(&s);
+    if (rc) return rc;
+    net->spec = s;
+    regime_gains(regime, thal_scale, &net->d.ge, &net->d.gi);
+    net->gen += 1;
     return G4S_OK;
 }
 
@@ -421,7 +607,9 @@ int g4s_destroy(g4s_net *net)
 {
     if (!net) return G4S_OK;
     if (net->stream) hipStreamSynchronize(net->stream);
+    drop_graph(net);
     for (void *p : net->allocs) hipFree(p);
+    if (net->h_tinfo) hipHostFree(net->h_tinfo);
     if (net->h_raster) hipHostFree(net->h_raster);
     if (net->h_ctl) hipHostFree(net->h_ctl);
     if (net->ev0) hipEventDestroy(net->ev0);
@@ -464,3 +652,90 @@ int g4s_device_normal_words(const uint32_t *x0, const uint32_t *x1, int64_t n, f
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Autotuner: scheme x tiling candidates timed on the live network, cached
+// per (log2 N, Nsyn, log2 rate, accum) bucket.
+// ---------------------------------------------------------------------------
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <tuple>
+
+namespace {
+
+using Bucket = std::tuple<int, int, int, int>;
+std::mutex g_tune_mu;
+std::map<Bucket, g4s_plan> g_tune_cache;
+
+int ilog2(double x) { return x <= 0 ? -100 : (int)std::floor(std::log2(x)); }
+
+// Time `plan` from a reset network: warm-up then `steps` steps.
+int time_plan(g4s_net *net, g4s_plan *plan, int steps)
+{
+    int rc = g4s_reset(net);
+    if (rc) return rc;
+    const int warm = std::max(10, steps / 4);
+    if ((rc = g4s_run(net, plan->alg, plan->accum, warm, nullptr, nullptr, &plan->opts))) return rc;
+    if ((rc = g4s_run(net, plan->alg, plan->accum, steps, nullptr, nullptr, &plan->opts))) return rc;
+    plan->us_per_step = (float)(net->timing.device_ms * 1000.0 / steps);
+    plan->rate_hz = (float)((double)net->timing.spikes / steps / (double)net->d.n * 1000.0);
+    return G4S_OK;
+}
+
+}  // namespace
+
+extern "C" int g4s_autotune(g4s_net *net, int accum, int32_t trial_steps, int force, g4s_plan *best)
+{
+    if (!net || !best) return fail(G4S_ERR_ARG, "NULL argument");
+    if (accum != G4S_ACCUM_ATOMIC && accum != G4S_ACCUM_DET) return fail(G4S_ERR_ARG, "unknown accum %d", accum);
+    if (trial_steps < 1) trial_steps = 100;
+    // probe the firing rate with a default plan to find the bucket
+    g4s_plan probe{};
+    probe.alg = accum == G4S_ACCUM_ATOMIC ? G4S_ALG_AB : G4S_ALG_N;
+    probe.accum = accum;
+    int rc = time_plan(net, &probe, std::max(20, trial_steps / 2));
+    if (rc) return rc;
+    const Bucket key{ilog2((double)net->d.n), net->d.nsyn, ilog2(probe.rate_hz), accum};
+    if (!force) {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            *best = it->second;
+            return g4s_reset(net);
+        }
+    }
+    std::vector<g4s_plan> cands;
+    for (int alg = G4S_ALG_AP; alg <= G4S_ALG_SKL; ++alg) {
+        const bool slice = accum == G4S_ACCUM_ATOMIC && (alg == G4S_ALG_AB || alg == G4S_ALG_SKL);
+        if (slice) {
+            for (int P : {64, 128, 256, 512})
+                for (int B : {256, 512, 1024}) {
+                    g4s_plan c{};
+                    c.alg = alg; c.accum = accum; c.opts.slices = P; c.opts.block = B;
+                    cands.push_back(c);
+                }
+        } else {
+            for (int B : {128, 256}) {
+                g4s_plan c{};
+                c.alg = alg; c.accum = accum; c.opts.block = B;
+                cands.push_back(c);
+            }
+        }
+    }
+    g4s_plan win{};
+    win.us_per_step = INFINITY;
+    for (g4s_plan &c : cands) {
+        rc = time_plan(net, &c, trial_steps);
+        if (rc == G4S_ERR_UNSUPPORTED || rc == G4S_ERR_ARG) continue;  // tiling not valid here
+        if (rc) return rc;
+        if (c.us_per_step < win.us_per_step) win = c;
+    }
+    if (!(win.us_per_step < INFINITY)) return fail(G4S_ERR_UNSUPPORTED, "no candidate plan ran");
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune_cache[key] = win;
+    }
+    *best = win;
+    return g4s_reset(net);
+}

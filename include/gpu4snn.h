@@ -56,10 +56,12 @@ typedef struct g4s_net g4s_net;
 
 /* Tuning of one run; zero-initialised fields select the defaults. */
 typedef struct g4s_run_opts {
-    int32_t block;       /* threads per workgroup (multiple of 64), 0 = 256            */
-    int32_t grid_waves;  /* N/S schemes: waves launched per step, 0 = from last count  */
+    int32_t block;       /* threads per workgroup (multiple of 64; <= 256, or <= 1024
+                            for AB/SKL atomic), 0 = chosen per scheme              */
+    int32_t slices;      /* AB/SKL atomic: target slices P (workgroups), 0 = auto   */
     int32_t chunk;       /* steps per device chunk between host copies, 0 = 1000       */
-    int32_t graph;       /* 1 = replay per-step schemes from a captured HIP graph      */
+    int32_t graph;       /* AB atomic: >= 0 replays the chunk's per-step launches from
+                            a captured HIP graph, < 0 launches them directly        */
 } g4s_run_opts;
 
 /* Timing of the last g4s_run call. */
@@ -94,8 +96,31 @@ int g4s_run(g4s_net *net, int alg, int accum, int32_t steps,
 
 int g4s_get_timing(const g4s_net *net, g4s_timing *out);
 
+/* Change the firing regime / thalamic scale of an existing network without
+ * rebuilding its connectivity: the reference's per-run `mode` argument of
+ * stateupdate (cu:344-347, cu:362-374, loop cu:1082). */
+int g4s_set_regime(g4s_net *net, int regime, float thal_scale);
+
 /* Current step index (number of steps simulated since the last reset). */
 int64_t g4s_current_step(const g4s_net *net);
+
+/* A propagation plan: scheme, accumulation mode and tiling. */
+typedef struct g4s_plan {
+    int32_t alg;          /* G4S_ALG_*                                        */
+    int32_t accum;        /* G4S_ACCUM_*                                      */
+    g4s_run_opts opts;    /* tiling (block, slices), chunk, graph             */
+    float   us_per_step;  /* measured device time of the plan                 */
+    float   rate_hz;      /* firing rate seen while measuring                 */
+} g4s_plan;
+
+/* Autotuner (no reference counterpart; the reference fixes its launch shapes
+ * by occupancy queries, cu:780-838, cu:957-1079).  Times every scheme and
+ * tiling candidate for `accum` on this network (trial_steps steps each after
+ * a short warm-up) and returns the fastest.  Plans are cached per bucket of
+ * (log2 N, Nsyn, log2 firing rate, accum) for the life of the process; a
+ * bucket hit returns the cached plan without timing unless `force` is set.
+ * The network is reset (step 0) on return. */
+int g4s_autotune(g4s_net *net, int accum, int32_t trial_steps, int force, g4s_plan *best);
 
 /* Copy the structure-of-arrays state out (any pointer may be NULL).  I is the
  * total input of the last simulated step; replaces the AoS Neuron copy-back

@@ -1,0 +1,151 @@
+"""GPU: scheme switching, chunking, edge sizes and larger networks.
+
+Deterministic mode is compared with the oracle bit for bit wherever the
+oracle runs in seconds; at N=100,000 the five schemes must agree with each
+other bit for bit (and with the oracle over a short horizon), and atomic mode
+must keep the firing rate within 1% of the deterministic run.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ALGS = ["AP", "N", "S", "AB", "SKL"]
+
+
+def oracle_raster(oracle, n, nsyn, steps, seed, regime=1, scale=1.0, v_init=None):
+    net = oracle.Network(n, nsyn, seed, regime=regime, thal_scale=scale)
+    if v_init is not None:
+        net.v[:] = v_init
+    r, c, last = net.run(steps)
+    return net, r, c
+
+
+def dense(g4s, words, n):
+    return g4s.raster_to_dense(words, n)
+
+
+@pytest.mark.parametrize("n,nsyn", [(2, 1), (65, 64), (130, 7), (1000, 999), (4099, 100)])
+def test_det_edge_sizes(g4s, oracle, n, nsyn):
+    onet, oraster, ocounts = oracle_raster(oracle, n, nsyn, 300, seed=n + nsyn, regime=2)
+    for alg in ALGS:
+        with g4s.Network(n, nsyn, n + nsyn, regime=2) as net:
+            words, counts = net.run(300, alg=alg, accum="det")
+            assert np.array_equal(dense(g4s, words, n), oraster), (alg, n, nsyn)
+            v, u, _, nos = net.state()
+            assert np.array_equal(v.view(np.uint32), onet.v.view(np.uint32)), alg
+            assert np.array_equal(nos, onet.nospks), alg
+
+
+def test_det_switching_schemes_and_chunks(g4s, oracle):
+    n, nsyn, seed = 3000, 300, 5
+    onet, oraster, _ = oracle_raster(oracle, n, nsyn, 700, seed)
+    plan = [("SKL", 100, 0), ("N", 50, 7), ("AP", 150, 0), ("S", 100, 33), ("AB", 120, 0), ("SKL", 180, 11)]
+    with g4s.Network(n, nsyn, seed) as net:
+        rows = []
+        for alg, steps, chunk in plan:
+            w, _ = net.run(steps, alg=alg, accum="det", chunk=chunk)
+            rows.append(dense(g4s, w, n))
+        assert np.array_equal(np.concatenate(rows), oraster)
+        v, u, _, _ = net.state()
+        assert np.array_equal(u.view(np.uint32), onet.u.view(np.uint32))
+
+
+def test_atomic_switching_keeps_pending_input(g4s, oracle):
+    """Switching among schemes that keep the pending input in HBM (N, S, AP)
+    and those that derive it from the queue (AB, SKL) or bitmap (det) must
+    not lose or double the input of the step at the switch."""
+    n, nsyn, seed = 2500, 1000, 9
+    _, oraster, _ = oracle_raster(oracle, n, nsyn, 100, seed)
+    plan = [("N", 10), ("SKL", 10), ("S", 10), ("AB", 10), ("AP", 10), ("SKL", 10), ("N", 10), ("AP", 10),
+            ("S", 10), ("AB", 10)]
+    with g4s.Network(n, nsyn, seed) as net:
+        rows = [dense(g4s, net.run(k, alg=a, accum="atomic")[0], n) for a, k in plan]
+    got = np.concatenate(rows)
+    assert np.array_equal(got, oraster)
+
+
+@pytest.mark.parametrize("graph,chunk", [(-1, 0), (0, 37), (0, 0)])
+def test_ab_graph_replay_and_chunks(g4s, oracle, graph, chunk):
+    """AB per-step launches replayed from a captured HIP graph (captured again
+    for a new chunk length) or launched directly give the same dynamics."""
+    n, nsyn, seed = 2500, 1000, 31
+    _, oraster, ocounts = oracle_raster(oracle, n, nsyn, 300, seed)
+    with g4s.Network(n, nsyn, seed) as net:
+        a = dense(g4s, net.run(100, alg="AB", graph=graph, chunk=chunk)[0], n)
+        w, c = net.run(200, alg="AB", graph=graph, chunk=chunk)
+    assert np.array_equal(a, oraster[:100])
+    total = a.sum() + c.sum()
+    assert abs(total - ocounts.sum()) <= 0.01 * ocounts.sum()
+
+
+def test_det_then_atomic_rebuilds_input(g4s, oracle):
+    n, nsyn, seed = 2500, 1000, 13
+    _, oraster, _ = oracle_raster(oracle, n, nsyn, 80, seed)
+    with g4s.Network(n, nsyn, seed) as net:
+        a = dense(g4s, net.run(40, alg="SKL", accum="det")[0], n)
+        b = dense(g4s, net.run(40, alg="N", accum="atomic")[0], n)
+    assert np.array_equal(np.concatenate([a, b]), oraster)
+
+
+def test_nan_injection_spikes_and_resets(g4s, oracle):
+    n, nsyn, seed = 256, 10, 3
+    v0 = np.full(n, -65.0, np.float32)
+    v0[[0, 77, 255]] = np.nan
+    onet, oraster, _ = oracle_raster(oracle, n, nsyn, 20, seed, v_init=v0)
+    for alg in ALGS:
+        for accum in ("det", "atomic"):
+            with g4s.Network(n, nsyn, seed) as net:
+                net.set_state(v=v0)
+                words, _ = net.run(20, alg=alg, accum=accum)
+                d = dense(g4s, words, n)
+                assert d[0, [0, 77, 255]].all(), (alg, accum)
+                if accum == "det":
+                    assert np.array_equal(d, oraster), alg
+
+
+def test_quiet_regime_and_zero_steps(g4s, oracle):
+    n, nsyn, seed = 2500, 1000, 21
+    _, oraster, ocounts = oracle_raster(oracle, n, nsyn, 500, seed, regime=0)
+    with g4s.Network(n, nsyn, seed, regime=0) as net:
+        w, c = net.run(0, alg="SKL")
+        assert w.shape[0] == 0 and net.t == 0
+        w, c = net.run(500, alg="SKL", accum="det")
+        assert np.array_equal(dense(g4s, w, n), oraster) and c.sum() == ocounts.sum()
+
+
+@pytest.fixture(scope="module")
+def big():
+    """N=100,000, Nsyn=1000 (configuration 1), balanced regime."""
+    return dict(n=100_000, nsyn=1000, seed=20221)
+
+
+def test_cfg1_det_all_schemes_identical_and_oracle(g4s, oracle, big):
+    n, nsyn, seed = big["n"], big["nsyn"], big["seed"]
+    steps = 60
+    onet, oraster, _ = oracle_raster(oracle, n, nsyn, steps, seed)
+    with g4s.Network(n, nsyn, seed) as net:
+        post, w = net.synapses()
+        assert np.array_equal(post, onet.post) and np.array_equal(w.view(np.uint32), onet.w.view(np.uint32))
+        ref = None
+        for alg in ALGS:
+            net.reset()
+            words, counts = net.run(steps, alg=alg, accum="det")
+            d = dense(g4s, words, n)
+            assert np.array_equal(d, oraster), alg
+            if ref is None:
+                ref = words
+            assert np.array_equal(words, ref), alg
+        v, u, _, _ = net.state()
+        assert np.array_equal(v.view(np.uint32), onet.v.view(np.uint32))
+
+
+def test_cfg1_atomic_rates(g4s, big):
+    n, nsyn, seed = big["n"], big["nsyn"], big["seed"]
+    with g4s.Network(n, nsyn, seed) as net:
+        _, cdet = net.run(1000, alg="SKL", accum="det")
+        ref = cdet.sum()
+        for alg in ALGS:
+            net.reset()
+            _, c = net.run(1000, alg=alg, accum="atomic")
+            assert abs(c.sum() - ref) <= 0.01 * ref, (alg, c.sum(), ref)
