@@ -33,6 +33,28 @@ def step_bytes(n, nsyn, spikes):
     return 40.0 * n + (4.0 + 8.0 * nsyn) * spikes
 
 
+def profiled_traffic(kernel, alg, accum, n, nsyn, regime, slices, block):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary of the same workload and plan, or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    key = f"{kernel}|{alg}|{accum}|n={n}|nsyn={nsyn}|regime={regime}|slices={slices}|block={block}"
+    try:
+        with open(path) as f:
+            rec = json.load(f).get(key)
+    except (OSError, ValueError):
+        return None
+    return None if rec is None else rec["bytes_per_launch"]
+
+
+def kernel_bytes(alg, accum, n, nsyn, spikes):
+    """Algorithmic bytes of the dominant kernel for a step with `spikes`
+    spikes: the whole step, except for N/S atomic whose separate update
+    kernel carries the 40 B per neuron."""
+    if accum == "atomic" and alg in ("N", "S"):
+        return (4.0 + 8.0 * nsyn) * spikes
+    return step_bytes(n, nsyn, spikes)
+
+
 def cpu_baseline(n, nsyn, seed, regime, steps):
     """The CPU oracle (single thread, fp32, -O2 -ffp-contract=off) on the first
     `steps` steps of the same network; connectivity build excluded."""
@@ -57,14 +79,30 @@ def main():
     ap.add_argument("--nsyn", type=int, default=1000)
     ap.add_argument("--seed", type=int, default=20221)
     ap.add_argument("--regime", type=int, default=1)
-    ap.add_argument("--alg", default="auto", help="scheme, or 'auto' for the autotuner's pick")
+    # Default plan: what g4s_autotune picks for this bucket on MI355X
+    # (AB atomic, 256 slices, 1024 threads); --alg auto re-tunes in-process.
+    ap.add_argument("--alg", default="AB", help="scheme, or 'auto' for the autotuner's pick")
     ap.add_argument("--accum", default="atomic")
-    ap.add_argument("--slices", type=int, default=0)
-    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--slices", type=int, default=256)
+    ap.add_argument("--block", type=int, default=1024)
     ap.add_argument("--sweep", action="store_true", help="also time every scheme x mode (stderr)")
-    ap.add_argument("--cpu-steps", type=int, default=300)
+    ap.add_argument("--cpu-steps", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device 5 times with a plain copy kernel "
+                         "(reference reading for the FETCH_SIZE/WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+
+    if args.copy_calibrate > 0:
+        import torch
+        nbytes = args.copy_calibrate << 20
+        a = torch.ones(nbytes // 4, dtype=torch.int32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": nbytes, "launches": 5}), flush=True)
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -112,12 +150,18 @@ def main():
     tm = net.timing()
     spikes = int(counts.sum())
     events = spikes * args.nsyn
-    dev_s = tm["device_ms"] / 1e3
-    total_bytes = sum(step_bytes(args.n, args.nsyn, int(x)) for x in counts)
-    launches = max(1, tm["launches"])
-    per_launch_bytes = total_bytes / launches
-    avg_launch_s = dev_s / launches
+
+    # Roofline of the dominant kernel: the same number of steps again, each
+    # launch of that kernel bracketed by HIP events on the library's stream.
+    _, kc = net.run(args.steps, alg=alg, accum=args.accum, raster=False, block=block, slices=slices,
+                    kernel_events=True)
+    tk = net.timing()
+    kb = sum(kernel_bytes(alg, args.accum, args.n, args.nsyn, int(x)) for x in kc)
+    k_launches = max(1, tk["kernel_launches"])
+    per_launch_bytes = kb / k_launches
+    avg_launch_s = tk["kernel_ms"] / 1e3 / k_launches
     achieved = per_launch_bytes / avg_launch_s / 1e9
+    kname = KERNEL_OF.get((alg, args.accum), "step")
 
     out = {
         "metric": "synaptic events/s + ms per sim-second, N=100k; p50/p99 req latency (served)",
@@ -140,9 +184,11 @@ def main():
                    "autotune": plan_info, "spikes_per_step": spikes / args.steps,
                    "firing_rate_hz": spikes / args.steps / args.n * 1000.0, "seed": args.seed},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": KERNEL_OF.get((alg, args.accum), "step"),
-                     "bytes_per_launch": per_launch_bytes, "avg_launch_ms": avg_launch_s * 1e3},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": profiled_traffic(kname, alg, args.accum, args.n, args.nsyn, args.regime, slices, block),
+                     "kernel": kname,
+                     "bytes_per_launch": per_launch_bytes, "avg_launch_us": avg_launch_s * 1e6,
+                     "launches_timed": k_launches},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.n, args.nsyn, args.seed, args.regime, args.cpu_steps)

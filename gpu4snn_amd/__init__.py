@@ -36,7 +36,8 @@ class Spec(C.Structure):
 
 
 class RunOpts(C.Structure):
-    _fields_ = [("block", C.c_int32), ("slices", C.c_int32), ("chunk", C.c_int32), ("graph", C.c_int32)]
+    _fields_ = [("block", C.c_int32), ("slices", C.c_int32), ("chunk", C.c_int32), ("graph", C.c_int32),
+                ("kernel_events", C.c_int32)]
 
 
 class Plan(C.Structure):
@@ -51,7 +52,8 @@ class Plan(C.Structure):
 
 class Timing(C.Structure):
     _fields_ = [("device_ms", C.c_double), ("wall_ms", C.c_double), ("steps", C.c_int64),
-                ("spikes", C.c_int64), ("launches", C.c_int32), ("grid", C.c_int32)]
+                ("spikes", C.c_int64), ("launches", C.c_int32), ("grid", C.c_int32), ("kernel_ms", C.c_double),
+                ("kernel_launches", C.c_int32), ("reserved", C.c_int32)]
 
 
 _vp = C.c_void_p
@@ -177,13 +179,14 @@ class Network:
     def reset(self):
         _check(lib().g4s_reset(self._h))
 
-    def run(self, steps, alg="SKL", accum="atomic", raster=True, block=0, chunk=0, graph=0, slices=0):
+    def run(self, steps, alg="SKL", accum="atomic", raster=True, block=0, chunk=0, graph=0, slices=0,
+            kernel_events=False):
         """Simulate `steps` steps; returns (raster words [steps, words] or None, counts)."""
         a = ALG[alg] if isinstance(alg, str) else int(alg)
         m = ACCUM[accum] if isinstance(accum, str) else int(accum)
         r = np.zeros((steps, words_of(self.n)), np.uint32) if raster else None
         counts = np.zeros(steps, np.int32)
-        opts = RunOpts(block, slices, chunk, graph)
+        opts = RunOpts(block, slices, chunk, graph, int(bool(kernel_events)))
         _check(lib().g4s_run(self._h, a, m, int(steps), _ptr(r), _ptr(counts), C.byref(opts)))
         return r, counts
 

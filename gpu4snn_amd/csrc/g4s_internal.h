@@ -54,6 +54,10 @@ struct DevNet {
 struct Launch {
     int grid, block;
     hipStream_t stream;
+    // Optional: events stamped by the dispatch itself at the kernel's start
+    // and end (hipExtLaunchKernelGGL) -- the dominant kernel's timing.
+    // Cooperative launches ignore them.
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
 };
 
 // Kernel launchers (g4s_kernels.hip).  Each returns the HIP status.

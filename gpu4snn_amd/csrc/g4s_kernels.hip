@@ -18,6 +18,7 @@
 #include "g4s_internal.h"
 
 #include <hip/hip_cooperative_groups.h>
+#include <hip/hip_ext.h>
 
 namespace g4s {
 
@@ -821,6 +822,18 @@ int grid_for(int64_t n, int block)
     return (int)g;
 }
 
+// Ordinary launch of `kernel`; with L's events set, the dispatch packet
+// stamps them at the kernel's start and end.
+template <typename... Args, typename F = void (*)(Args...)>
+hipError_t plain_launch(F kernel, const Launch &L, uint32_t shmem, Args... args)
+{
+    if (L.ev_start)
+        hipExtLaunchKernelGGL(kernel, dim3(L.grid), dim3(L.block), shmem, L.stream, L.ev_start, L.ev_stop, 0, args...);
+    else
+        hipLaunchKernelGGL(kernel, dim3(L.grid), dim3(L.block), shmem, L.stream, args...);
+    return hipGetLastError();
+}
+
 hipError_t coop(const void *fn, const Launch &L, void **args, size_t shmem = 0)
 {
     return hipLaunchCooperativeKernel(fn, dim3(L.grid), dim3(L.block), args, shmem, L.stream);
@@ -837,8 +850,7 @@ hipError_t launch_skl_slice_r(const DevNet &d, int t0, int steps, int last_t, co
 template <int R>
 hipError_t launch_slice_step_r(const DevNet &d, int rel, const Launch &L)
 {
-    hipLaunchKernelGGL(k_slice_step<R>, dim3(L.grid), dim3(L.block), slice_lds_bytes(d.L, L.block), L.stream, d, rel);
-    return hipGetLastError();
+    return plain_launch(k_slice_step<R>, L, (uint32_t)slice_lds_bytes(d.L, L.block), d, rel);
 }
 
 const void *skl_slice_fn(int per_thread)
@@ -868,14 +880,12 @@ hipError_t launch_fill_u32(uint32_t *p, size_t n, uint32_t val, hipStream_t s)
 
 hipError_t launch_update(const DevNet &d, int t, int rel, int last, const Launch &L)
 {
-    hipLaunchKernelGGL(k_update, dim3(L.grid), dim3(L.block), 0, L.stream, d, t, rel, last);
-    return hipGetLastError();
+    return plain_launch(k_update, L, 0, d, t, rel, last);
 }
 
 hipError_t launch_push_rows(const DevNet &d, int t, int rel, const Launch &L)
 {
-    hipLaunchKernelGGL(k_push_rows, dim3(L.grid), dim3(L.block), 0, L.stream, d, t, rel);
-    return hipGetLastError();
+    return plain_launch(k_push_rows, L, 0, d, t, rel);
 }
 
 hipError_t launch_push_bitmap(const DevNet &d, int t, hipStream_t s)
@@ -887,14 +897,14 @@ hipError_t launch_push_bitmap(const DevNet &d, int t, hipStream_t s)
 
 hipError_t launch_reduce_counts(const DevNet &d, int steps, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_reduce_counts, dim3(grid_for(steps, 256)), dim3(256), 0, s, d, steps);
+    // one workgroup per step
+    hipLaunchKernelGGL(k_reduce_counts, dim3(steps < 4096 ? (steps > 0 ? steps : 1) : 4096), dim3(256), 0, s, d, steps);
     return hipGetLastError();
 }
 
 hipError_t launch_push_flat(const DevNet &d, int t, int rel, const Launch &L)
 {
-    hipLaunchKernelGGL(k_push_flat, dim3(L.grid), dim3(L.block), 0, L.stream, d, t, rel);
-    return hipGetLastError();
+    return plain_launch(k_push_flat, L, 0, d, t, rel);
 }
 
 hipError_t launch_ap(const DevNet &d, int t, int rel, int last, int prev_count, const Launch &L)
@@ -945,10 +955,7 @@ int skl_slice_blocks_per_cu(int L, int block)
 
 hipError_t launch_pull_thread(const DevNet &d, int t, int rel, int last, const Launch &L, bool cooperative)
 {
-    if (!cooperative) {
-        hipLaunchKernelGGL(k_pull_thread, dim3(L.grid), dim3(L.block), 0, L.stream, d, t, rel, last);
-        return hipGetLastError();
-    }
+    if (!cooperative) return plain_launch(k_pull_thread, L, 0, d, t, rel, last);
     DevNet dd = d;
     void *args[] = {&dd, &t, &rel, &last};
     return coop((const void *)k_pull_thread, L, args);
@@ -956,8 +963,7 @@ hipError_t launch_pull_thread(const DevNet &d, int t, int rel, int last, const L
 
 hipError_t launch_pull_wave(const DevNet &d, int t, int rel, int last, const Launch &L)
 {
-    hipLaunchKernelGGL(k_pull_wave, dim3(L.grid), dim3(L.block), 0, L.stream, d, t, rel, last);
-    return hipGetLastError();
+    return plain_launch(k_pull_wave, L, 0, d, t, rel, last);
 }
 
 hipError_t launch_ap_det(const DevNet &d, int t, int rel, int last, const Launch &L)

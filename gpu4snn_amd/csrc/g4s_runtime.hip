@@ -75,6 +75,11 @@ struct g4s_net {
     int cus = 0;
     g4s_timing timing{};
     std::vector<void *> allocs;
+    // Event pairs bracketing each launch of the dominant kernel of a chunk
+    // (g4s_run_opts.kernel_events); grown on demand, destroyed with the net.
+    std::vector<hipEvent_t> kev;
+    bool kev_on = false;
+    int kev_used = 0;
 
     template <class T>
     int alloc(T **p, size_t count)
@@ -87,6 +92,55 @@ struct g4s_net {
 };
 
 namespace {
+
+// Grow the event pool so that two more events are available.
+hipError_t kev_reserve(g4s_net *net)
+{
+    while ((size_t)net->kev_used + 2 > net->kev.size()) {
+        hipEvent_t e = nullptr;
+        hipError_t r = hipEventCreate(&e);
+        if (r != hipSuccess) return r;
+        net->kev.push_back(e);
+    }
+    return hipSuccess;
+}
+
+// Cooperative launches: record the opening (begin) or closing event of one
+// dominant-kernel launch on the stream.
+hipError_t kev_mark(g4s_net *net, bool begin)
+{
+    if (!net->kev_on) return hipSuccess;
+    if (begin) {
+        hipError_t r = kev_reserve(net);
+        if (r != hipSuccess) return r;
+    }
+    return hipEventRecord(net->kev[net->kev_used++], net->stream);
+}
+
+// Ordinary launches: hand the next event pair to the dispatch itself.
+hipError_t kev_stamp(g4s_net *net, Launch *L)
+{
+    L->ev_start = L->ev_stop = nullptr;
+    if (!net->kev_on) return hipSuccess;
+    hipError_t r = kev_reserve(net);
+    if (r != hipSuccess) return r;
+    L->ev_start = net->kev[net->kev_used++];
+    L->ev_stop = net->kev[net->kev_used++];
+    return hipSuccess;
+}
+
+// Sum of the bracketed kernel durations of the chunk (after a stream sync).
+hipError_t kev_collect(g4s_net *net, double *ms)
+{
+    for (int k = 0; k + 1 < net->kev_used; k += 2) {
+        float x = 0.0f;
+        hipError_t r = hipEventElapsedTime(&x, net->kev[k], net->kev[k + 1]);
+        if (r != hipSuccess) return r;
+        *ms += x;
+    }
+    net->kev_used = 0;
+    return hipSuccess;
+}
 
 int ensure_chunk(g4s_net *net, int chunk)
 {
@@ -244,7 +298,11 @@ int run_ab_chunk(g4s_net *net, int cs, bool use_graph, int block)
 {
     const Launch L{net->d.P, block, net->stream};
     if (!use_graph) {
-        for (int rel = 0; rel < cs; ++rel) HIP_OK(launch_slice_step(net->d, rel, L));
+        for (int rel = 0; rel < cs; ++rel) {
+            Launch Lk = L;
+            HIP_OK(kev_stamp(net, &Lk));
+            HIP_OK(launch_slice_step(net->d, rel, Lk));
+        }
         return G4S_OK;
     }
     if (!net->graph_exec || net->graph_steps != cs || net->graph_gen != net->gen || net->graph_block != block) {
@@ -375,36 +433,48 @@ hipError_t launch_step(g4s_net *net, int alg, int accum, int block, int t, int r
 {
     DevNet &d = net->d;
     *rc = G4S_OK;
+    hipError_t e;
     if (accum == G4S_ACCUM_DET) {
         *launches += 1;
         switch (alg) {
         case G4S_ALG_AP:
             if ((*rc = coop_launch(net, kCoopAPDet, block, d.n, L))) return hipSuccess;
-            return launch_ap_det(d, t, rel, last, *L);
+            if ((e = kev_mark(net, true)) != hipSuccess) return e;
+            e = launch_ap_det(d, t, rel, last, *L);
+            break;
         case G4S_ALG_N:
             *L = plain(net, d.n, block);
+            if ((e = kev_stamp(net, L)) != hipSuccess) return e;
             return launch_pull_thread(d, t, rel, last, *L, false);
         case G4S_ALG_S:
             *L = plain(net, ((int64_t)d.n + 63) / 64 * 64, block);
+            if ((e = kev_stamp(net, L)) != hipSuccess) return e;
             return launch_pull_wave(d, t, rel, last, *L);
         default:  // AB
             if ((*rc = coop_launch(net, kCoopPull, block, d.n, L))) return hipSuccess;
-            return launch_pull_thread(d, t, rel, last, *L, true);
+            if ((e = kev_mark(net, true)) != hipSuccess) return e;
+            e = launch_pull_thread(d, t, rel, last, *L, true);
+            break;
         }
+        return e != hipSuccess ? e : kev_mark(net, false);
     }
     if (alg == G4S_ALG_AP) {
         *launches += 1;
         if ((*rc = coop_launch(net, kCoopAP, block, d.n, L))) return hipSuccess;
-        return launch_ap(d, t, rel, last, net->last_count, *L);
+        if ((e = kev_mark(net, true)) != hipSuccess) return e;
+        e = launch_ap(d, t, rel, last, net->last_count, *L);
+        return e != hipSuccess ? e : kev_mark(net, false);
     }
     *launches += 2;
-    hipError_t e = launch_update(d, t, rel, last, plain(net, d.n, block));
+    e = launch_update(d, t, rel, last, plain(net, d.n, block));
     if (e != hipSuccess) return e;
     if (alg == G4S_ALG_N) {
         *L = plain(net, net->est_count * 64, block);
+        if ((e = kev_stamp(net, L)) != hipSuccess) return e;
         return launch_push_rows(d, t, rel, *L);
     }
     *L = plain(net, net->est_count * (int64_t)d.nsyn, block);
+    if ((e = kev_stamp(net, L)) != hipSuccess) return e;
     return launch_push_flat(d, t, rel, *L);
 }
 
@@ -461,12 +531,16 @@ int g4s_run(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, i
         HIP_OK(launch_fill_u32(d.q0, (size_t)d.n, kSentinel, net->stream));
         HIP_OK(launch_fill_u32(d.q1, (size_t)d.n, kSentinel, net->stream));
     }
-    const bool use_graph = o.graph >= 0;
+    // Per-launch event brackets need separate launches: they replace the
+    // graph replay of AB steps for this run.
+    net->kev_on = o.kernel_events != 0;
+    net->kev_used = 0;
+    const bool use_graph = o.graph >= 0 && !net->kev_on;
 
     const auto wall0 = std::chrono::steady_clock::now();
-    double dev_ms = 0.0;
+    double dev_ms = 0.0, kern_ms = 0.0;
     int64_t spikes = 0;
-    int launches = 0, dom_grid = 0;
+    int launches = 0, dom_grid = 0, kern_launches = 0;
     const int64_t t_end = net->t + steps;
     const int64_t words = d.words;
 
@@ -481,6 +555,7 @@ int g4s_run(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, i
             HIP_OK(hipMemcpyAsync((void *)d.tinfo, net->h_tinfo, 2 * sizeof(int32_t), hipMemcpyHostToDevice, net->stream));
         }
         HIP_OK(hipEventRecord(net->ev0, net->stream));
+        const bool one_launch = alg == G4S_ALG_SKL;  // ev0..ev1 brackets exactly it
         if (alg == G4S_ALG_AB && slice) {
             if ((rc = run_ab_chunk(net, cs, use_graph, sblock))) return rc;
             launches += cs;
@@ -517,6 +592,15 @@ int g4s_run(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, i
         float ms = 0.0f;
         HIP_OK(hipEventElapsedTime(&ms, net->ev0, net->ev1));
         dev_ms += ms;
+        if (net->kev_on) {
+            if (one_launch) {
+                kern_ms += ms;
+                kern_launches += 1;
+            } else {
+                kern_launches += net->kev_used / 2;
+                HIP_OK(kev_collect(net, &kern_ms));
+            }
+        }
         int64_t mx = 0;
         for (int r = 0; r < cs; ++r) {
             const int32_t cnt = net->h_ctl[r * kCtlStride + kCtlCount];
@@ -535,6 +619,9 @@ int g4s_run(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, i
     net->timing.spikes = spikes;
     net->timing.launches = launches;
     net->timing.grid = dom_grid;
+    net->timing.kernel_ms = kern_ms;
+    net->timing.kernel_launches = kern_launches;
+    net->kev_on = false;
     return G4S_OK;
 }
 
@@ -612,6 +699,7 @@ int g4s_destroy(g4s_net *net)
     if (net->h_tinfo) hipHostFree(net->h_tinfo);
     if (net->h_raster) hipHostFree(net->h_raster);
     if (net->h_ctl) hipHostFree(net->h_ctl);
+    for (hipEvent_t e : net->kev) hipEventDestroy(e);
     if (net->ev0) hipEventDestroy(net->ev0);
     if (net->ev1) hipEventDestroy(net->ev1);
     if (net->stream) hipStreamDestroy(net->stream);

@@ -62,6 +62,9 @@ typedef struct g4s_run_opts {
     int32_t chunk;       /* steps per device chunk between host copies, 0 = 1000       */
     int32_t graph;       /* AB atomic: >= 0 replays the chunk's per-step launches from
                             a captured HIP graph, < 0 launches them directly        */
+    int32_t kernel_events; /* != 0: bracket every launch of the dominant kernel with
+                            HIP events (g4s_timing.kernel_ms); AB steps are then
+                            launched directly instead of from a graph              */
 } g4s_run_opts;
 
 /* Timing of the last g4s_run call. */
@@ -72,6 +75,10 @@ typedef struct g4s_timing {
     int64_t spikes;      /* spikes in those steps                                      */
     int32_t launches;    /* kernel launches issued                                     */
     int32_t grid;        /* workgroups of the dominant kernel                          */
+    double  kernel_ms;   /* with kernel_events: summed event-timed duration of the
+                            dominant kernel's launches, else 0                         */
+    int32_t kernel_launches; /* launches that kernel_ms covers                         */
+    int32_t reserved;
 } g4s_timing;
 
 /* Allocate device memory, build the connectivity table, initialise neurons

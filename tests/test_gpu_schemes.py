@@ -149,3 +149,38 @@ def test_cfg1_atomic_rates(g4s, big):
             net.reset()
             _, c = net.run(1000, alg=alg, accum="atomic")
             assert abs(c.sum() - ref) <= 0.01 * ref, (alg, c.sum(), ref)
+
+
+@pytest.mark.parametrize("alg", ALGS)
+def test_kernel_events_do_not_change_results(g4s, oracle, alg):
+    """Event-bracketed launches (roofline timing) simulate the same steps."""
+    n, nsyn, seed = 3000, 300, 77
+    _, oraster, _ = oracle_raster(oracle, n, nsyn, 200, seed)
+    with g4s.Network(n, nsyn, seed) as net:
+        words, _ = net.run(200, alg=alg, accum="det", kernel_events=True, chunk=64)
+        assert np.array_equal(dense(g4s, words, n), oraster), alg
+        tm = net.timing()
+        assert tm["kernel_launches"] >= 1 and tm["kernel_ms"] > 0.0
+        assert tm["kernel_ms"] <= tm["device_ms"] * 1.05 + 0.05
+        net.reset()
+        _, c = net.run(200, alg=alg, accum="atomic", kernel_events=True)
+        assert net.timing()["kernel_launches"] >= 1
+
+
+def test_autotune_plan_runs_and_is_cached(g4s, oracle):
+    n, nsyn, seed = 4000, 200, 5
+    _, oraster, _ = oracle_raster(oracle, n, nsyn, 100, seed)
+    with g4s.Network(n, nsyn, seed) as net:
+        p = net.autotune("det", trial_steps=30)
+        assert p.accum == 1 and 0 <= p.alg <= 4 and p.us_per_step > 0
+        assert net.t == 0
+        words, _ = net.run_plan(p, 100)
+        assert np.array_equal(dense(g4s, words, n), oraster)
+        pa = net.autotune("atomic", trial_steps=30)
+        assert pa.accum == 0 and pa.us_per_step > 0
+        q = net.autotune("atomic", trial_steps=30)   # bucket hit: cached plan
+        assert (q.alg, q.opts.block, q.opts.slices) == (pa.alg, pa.opts.block, pa.opts.slices)
+        assert q.us_per_step == pa.us_per_step
+        net.reset()
+        _, c = net.run_plan(pa, 300, raster=False)
+        assert c.sum() > 0
