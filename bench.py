@@ -23,7 +23,8 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 # dominant kernel of each (scheme, mode): the one launch that carries a step
 KERNEL_OF = {("SKL", "atomic"): "k_skl_slice", ("AB", "atomic"): "k_slice_step", ("AP", "atomic"): "k_ap",
-             ("SKL", "det"): "k_skl_det", ("AB", "det"): "k_pull_thread", ("N", "det"): "k_pull_thread",
+             ("N", "atomic"): "k_push_rows", ("S", "atomic"): "k_push_flat",
+             ("SKL", "det"): "k_skl_slice", ("AB", "det"): "k_slice_step", ("N", "det"): "k_pull_thread",
              ("S", "det"): "k_pull_wave", ("AP", "det"): "k_ap_det"}
 
 
@@ -70,6 +71,81 @@ def cpu_baseline(n, nsyn, seed, regime, steps):
             "ms_per_sim_second": dt * 1000.0 / steps * 1000.0}
 
 
+def batch_specs(count, seed):
+    """cfg3-style instances: N log-uniform on [1e3, 5e4], Nsyn from {100, 500,
+    1000}, independent seeds, thalamic scale U(0.8, 1.5)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(round(float(np.exp(rng.uniform(np.log(1000.0), np.log(50000.0))))))
+        nsyn = min(int(rng.choice([100, 500, 1000])), n - 1)
+        out.append((n, nsyn, seed * 100003 + i, float(rng.uniform(0.8, 1.5))))
+    return out
+
+
+def bench_batch(g, args):
+    specs = batch_specs(args.batch, args.seed)
+    total = args.warmup + args.steps
+    t0 = time.perf_counter()
+    b = g.Batch(args.accum, chunk=args.chunk)
+    slots = [b.add(n, ns, sd, total, thal_scale=sc, keep_raster=False) for (n, ns, sd, sc) in specs]
+    print(f"batch of {len(slots)} built in {time.perf_counter() - t0:.1f} s, "
+          f"{sum(n for n, _, _, _ in specs)} neurons, {sum(n * ns for n, ns, _, _ in specs) / 1e9:.2f}e9 synapses",
+          file=sys.stderr, flush=True)
+    done = 0
+    while done < args.warmup:
+        b.step()
+        done += args.chunk
+    c0 = [len(b.counts(s)) for s in slots]
+    t1 = time.perf_counter()
+    dev_ms, launches = 0.0, 0
+    while b.step():
+        tm = b.timing()
+        dev_ms += tm["device_ms"]
+        launches += tm["launches"]
+    wall = time.perf_counter() - t1
+    spikes = events = 0
+    byts = 0.0
+    for s, (n, ns, _, _) in zip(slots, specs):
+        c = b.counts(s)[c0[slots.index(s)]:]
+        spikes += int(c.sum())
+        events += int(c.sum()) * ns
+        byts += 40.0 * n * len(c) + (4.0 + 8.0 * ns) * float(c.sum())
+    steps_timed = total - c0[0]
+    b.close()
+    sim_s = len(specs) * steps_timed / 1000.0
+    # back-to-back plain runs of a sample, one network per handle (AB scheme)
+    idx = list(range(0, len(specs), max(1, len(specs) // max(1, args.plain_sample))))[:args.plain_sample]
+    p_wall, p_sim = 0.0, 0.0
+    for i in idx:
+        n, ns, sd, sc = specs[i]
+        with g.Network(n, ns, sd, thal_scale=sc) as net:
+            net.run(args.warmup, alg="AB", accum=args.accum, raster=False)
+            t2 = time.perf_counter()
+            net.run(args.steps, alg="AB", accum=args.accum, raster=False)
+            p_wall += time.perf_counter() - t2
+            p_sim += args.steps / 1000.0
+    achieved = byts / (dev_ms / 1e3) / 1e9 if dev_ms > 0 else 0.0
+    return {
+        "metric": "aggregate simulated seconds per second, batched networks",
+        "value": sim_s / wall, "unit": "sim-s/s", "n_gpus": 1, "steps": steps_timed, "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / steps_timed, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded Philox networks)",
+        "config": {"workload": f"{len(specs)} networks, N log-uniform [1e3, 5e4], Nsyn in {{100,500,1000}}, "
+                               f"thalamic x U(0.8,1.5), {steps_timed} timed steps each",
+                   "accum": args.accum, "chunk": args.chunk, "neurons": sum(x[0] for x in specs),
+                   "synapses": sum(x[0] * x[1] for x in specs), "seed": args.seed},
+        "events_per_s": events / wall,
+        "plain": {"sim_s_per_s": p_sim / p_wall, "sample": len(idx), "scheme": "AB"},
+        "speedup_vs_plain": (sim_s / wall) / (p_sim / p_wall),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_batch_step",
+                     "bytes_per_launch": byts / max(1, launches),
+                     "avg_launch_us": dev_ms * 1e3 / max(1, launches)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +164,12 @@ def main():
     ap.add_argument("--sweep", action="store_true", help="also time every scheme x mode (stderr)")
     ap.add_argument("--cpu-steps", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch", type=int, default=0, metavar="COUNT",
+                    help="batched form instead: COUNT independent networks (N log-uniform in [1e3, 5e4], "
+                         "Nsyn in {100, 500, 1000}, thalamic scale U(0.8, 1.5)) in one batch, compared "
+                         "with back-to-back plain runs of a sample of them")
+    ap.add_argument("--chunk", type=int, default=100)
+    ap.add_argument("--plain-sample", type=int, default=32)
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times with a plain copy kernel "
                          "(reference reading for the FETCH_SIZE/WRITE_SIZE counters) and exit")
@@ -111,6 +193,10 @@ def main():
         sys.exit(2)
 
     import gpu4snn_amd as g
+
+    if args.batch > 0:
+        print(json.dumps(bench_batch(g, args)), flush=True)
+        return
 
     t_build = time.perf_counter()
     net = g.Network(args.n, args.nsyn, args.seed, regime=args.regime)

@@ -11,7 +11,8 @@ import os
 
 import numpy as np
 
-__all__ = ["ALG", "ACCUM", "Network", "lib", "G4SError", "words_of", "raster_to_dense", "n_exc_of"]
+__all__ = ["ALG", "ACCUM", "Network", "Batch", "lib", "G4SError", "words_of", "raster_to_dense", "n_exc_of",
+           "footprint"]
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "libgpu4snn.so")
@@ -56,6 +57,17 @@ class Timing(C.Structure):
                 ("kernel_launches", C.c_int32), ("reserved", C.c_int32)]
 
 
+class BatchOpts(C.Structure):
+    _fields_ = [("chunk", C.c_int32), ("slice_len", C.c_int32), ("graph", C.c_int32), ("reserved", C.c_int32),
+                ("budget", C.c_int64)]
+
+
+class InstInfo(C.Structure):
+    _fields_ = [("n", C.c_int64), ("nsyn", C.c_int32), ("slice_len", C.c_int32), ("t", C.c_int64),
+                ("steps", C.c_int64), ("spikes", C.c_int64), ("bytes", C.c_int64), ("done", C.c_int32),
+                ("slices", C.c_int32)]
+
+
 _vp = C.c_void_p
 
 
@@ -84,6 +96,18 @@ def lib():
             "g4s_device_thalamic": ([C.c_uint64, C.c_uint32, C.c_int64, _vp], C.c_int),
             "g4s_device_normal_words": ([_vp, _vp, C.c_int64, _vp], C.c_int),
             "g4s_autotune": ([_vp, C.c_int, C.c_int32, C.c_int, C.POINTER(Plan)], C.c_int),
+            "g4s_batch_footprint": ([C.POINTER(Spec), C.c_int32, C.c_int32], C.c_int64),
+            "g4s_batch_create": ([C.c_int, C.POINTER(BatchOpts), C.POINTER(_vp)], C.c_int),
+            "g4s_batch_add": ([_vp, C.POINTER(Spec), C.c_int64, C.c_int, C.POINTER(C.c_int32)], C.c_int),
+            "g4s_batch_step": ([_vp, C.POINTER(C.c_int32)], C.c_int),
+            "g4s_batch_run": ([_vp], C.c_int),
+            "g4s_batch_info": ([_vp, C.c_int32, C.POINTER(InstInfo)], C.c_int),
+            "g4s_batch_counts": ([_vp, C.c_int32, _vp], C.c_int),
+            "g4s_batch_raster": ([_vp, C.c_int32, _vp], C.c_int),
+            "g4s_batch_state": ([_vp, C.c_int32, _vp, _vp, _vp, _vp], C.c_int),
+            "g4s_batch_remove": ([_vp, C.c_int32], C.c_int),
+            "g4s_batch_timing": ([_vp, C.POINTER(Timing)], C.c_int),
+            "g4s_batch_destroy": ([_vp], C.c_int),
         }
         for name, (args, res) in sigs.items():
             f = getattr(L, name)
@@ -227,3 +251,85 @@ class Network:
         w = np.empty(self.n * self.nsyn, np.float32)
         _check(lib().g4s_get_synapses(self._h, _ptr(post), _ptr(w)))
         return post, w
+
+
+def footprint(n, nsyn, chunk=0, slice_len=0, seed=0, regime=1, thal_scale=1.0):
+    """Device bytes of one batched instance (g4s_batch_footprint)."""
+    v = lib().g4s_batch_footprint(C.byref(make_spec(n, nsyn, seed, regime, thal_scale)), chunk, slice_len)
+    if v < 0:
+        raise G4SError(-1, lib().g4s_last_error().decode())
+    return v
+
+
+class Batch:
+    """Many device-resident networks advanced together (g4s_batch_*)."""
+
+    def __init__(self, accum="det", chunk=0, slice_len=0, graph=0, budget=0):
+        self._h = _vp()
+        self.accum = accum
+        opts = BatchOpts(chunk, slice_len, graph, 0, int(budget))
+        _check(lib().g4s_batch_create(ACCUM[accum], C.byref(opts), C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().g4s_batch_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def add(self, n, nsyn, seed, steps, regime=1, thal_scale=1.0, keep_raster=True, n_exc=-1):
+        slot = C.c_int32()
+        spec = make_spec(n, nsyn, seed, regime, thal_scale, n_exc)
+        _check(lib().g4s_batch_add(self._h, C.byref(spec), int(steps), int(bool(keep_raster)), C.byref(slot)))
+        return slot.value
+
+    def step(self):
+        """One chunk; returns the number of instances advanced."""
+        a = C.c_int32()
+        _check(lib().g4s_batch_step(self._h, C.byref(a)))
+        return a.value
+
+    def run(self):
+        _check(lib().g4s_batch_run(self._h))
+
+    def info(self, slot):
+        i = InstInfo()
+        _check(lib().g4s_batch_info(self._h, slot, C.byref(i)))
+        return {k: getattr(i, k) for k, _ in InstInfo._fields_}
+
+    def counts(self, slot):
+        t = self.info(slot)["t"]
+        c = np.zeros(t, np.int32)
+        _check(lib().g4s_batch_counts(self._h, slot, _ptr(c)))
+        return c
+
+    def raster(self, slot):
+        i = self.info(slot)
+        w = np.zeros((i["t"], words_of(i["n"])), np.uint32)
+        _check(lib().g4s_batch_raster(self._h, slot, _ptr(w)))
+        return w
+
+    def state(self, slot):
+        n = self.info(slot)["n"]
+        v, u, I = (np.empty(n, np.float32) for _ in range(3))
+        nos = np.empty(n, np.int32)
+        _check(lib().g4s_batch_state(self._h, slot, _ptr(v), _ptr(u), _ptr(I), _ptr(nos)))
+        return v, u, I, nos
+
+    def remove(self, slot):
+        _check(lib().g4s_batch_remove(self._h, slot))
+
+    def timing(self):
+        t = Timing()
+        _check(lib().g4s_batch_timing(self._h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in Timing._fields_}

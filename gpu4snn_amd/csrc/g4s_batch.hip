@@ -1,0 +1,561 @@
+// g4s_batch.hip -- batched form: many independent networks resident on one
+// device, advanced together by one launch per step (include/gpu4snn.h,
+// g4s_batch_*).
+//
+// Each instance is a network of its own (state, connectivity, slice offset
+// table) in one device allocation.  A chunk of `cs` steps is one launch of
+// k_batch_step per step over the work list {slot, slice} of every unfinished
+// instance, replayed from a captured HIP graph.  The kernels read the
+// instance descriptors and the work list from device memory, so the graph
+// stays valid while instances come and go; it is captured again only when
+// the chunk length, the padded grid, the LDS size or a pool address changes.
+// Raster rows (instances that keep them) and per-step counts are copied to
+// the host once per chunk.
+#include "g4s_build.h"
+#include "g4s_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+using namespace g4s;
+
+#define HIP_OK G4S_HIP_OK
+
+namespace {
+
+constexpr int kBlock = 256;          // threads per workgroup of the batched kernels
+constexpr int kDefaultChunk = 100;
+constexpr int kMaxSliceLen = 4096;
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+struct Layout {  // sub-allocation of one instance's device block
+    size_t v, u, a, b, c, d, lastI, nospks, bm0, bm1, syn, soff, total;
+};
+
+Layout layout_of(int64_t n, int32_t nsyn, int64_t P)
+{
+    Layout l{};
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = off;
+        off += align256(bytes);
+        return at;
+    };
+    const size_t words = (size_t)((n + 31) / 32);
+    l.v = take(n * 4); l.u = take(n * 4); l.a = take(n * 4); l.b = take(n * 4);
+    l.c = take(n * 4); l.d = take(n * 4); l.lastI = take(n * 4); l.nospks = take(n * 4);
+    l.bm0 = take(words * 4); l.bm1 = take(words * 4);
+    l.syn = take((size_t)n * (size_t)nsyn * 8);
+    l.soff = take((size_t)n * (size_t)(P + 1) * 2);
+    l.total = off;
+    return l;
+}
+
+// Default slice length: at most 512 neurons per workgroup (a multiple of 64).
+int default_slice_len(int64_t n)
+{
+    const int64_t r = (n + 63) / 64 * 64;
+    return (int)std::min<int64_t>(r, 512);
+}
+
+struct Inst {
+    bool used = false;
+    g4s_spec spec{};
+    DevNet d{};
+    void *block = nullptr;
+    int64_t bytes = 0;      // device block + this instance's share of the pools
+    int64_t t = 0, steps = 0, spikes = 0;
+    bool keep_raster = false;
+    std::vector<int32_t> counts;
+    std::vector<uint32_t> raster;
+    size_t ring_off = 0, bcnt_off = 0;  // in the pools, this chunk
+};
+
+}  // namespace
+
+struct g4s_batch {
+    int accum = G4S_ACCUM_ATOMIC;
+    g4s_batch_opts opts{};
+    int chunk = kDefaultChunk;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<Inst> inst;
+    int64_t bytes_in_use = 0;
+    // descriptor table and its pinned mirror
+    DevNet *d_desc = nullptr, *h_desc = nullptr;
+    int desc_cap = 0;
+    // work list, padded to the grid
+    uint2 *d_work = nullptr, *h_work = nullptr;
+    int work_cap = 0;
+    // chunk pools: raster rows, per-slice counts, per-instance counts
+    uint32_t *d_ring = nullptr, *h_ring = nullptr;
+    size_t ring_cap = 0;
+    int32_t *d_bcnt = nullptr;
+    size_t bcnt_cap = 0;
+    int32_t *d_slots = nullptr, *h_slots = nullptr;
+    int32_t *d_counts = nullptr, *h_counts = nullptr;
+    size_t counts_cap = 0;
+    int slots_cap = 0;
+    // graph of one chunk
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int g_cs = 0, g_grid = 0;
+    size_t g_lds = 0;
+    int64_t gen = 0, g_gen = -1;
+    g4s_timing timing{};
+};
+
+namespace {
+
+template <class T>
+int grow_dev(g4s_batch *b, T **d, T **h, size_t *cap, size_t need)
+{
+    if (need <= *cap) return G4S_OK;
+    size_t want = std::max<size_t>(need, *cap + *cap / 2);
+    want = std::max<size_t>(want, 64);
+    if (*d) hipFree(*d);
+    if (h && *h) hipHostFree(*h);
+    *d = nullptr;
+    if (h) *h = nullptr;
+    if (hipMalloc((void **)d, want * sizeof(T)) != hipSuccess) {
+        *cap = 0;
+        return fail(G4S_ERR_MEM, "batch pool of %zu bytes", want * sizeof(T));
+    }
+    if (h && hipHostMalloc((void **)h, want * sizeof(T)) != hipSuccess) {
+        *cap = 0;
+        return fail(G4S_ERR_MEM, "pinned staging of %zu bytes", want * sizeof(T));
+    }
+    *cap = want;
+    b->gen += 1;
+    return G4S_OK;
+}
+
+int grow_int(g4s_batch *b, int32_t **d, int32_t **h, int *cap, int need)
+{
+    size_t c = (size_t)*cap;
+    int rc = grow_dev(b, d, h, &c, (size_t)need);
+    *cap = (int)c;
+    return rc;
+}
+
+void drop_graph(g4s_batch *b)
+{
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+int check_slot(const g4s_batch *b, int32_t slot)
+{
+    if (!b) return fail(G4S_ERR_ARG, "batch is NULL");
+    if (slot < 0 || slot >= (int32_t)b->inst.size() || !b->inst[(size_t)slot].used)
+        return fail(G4S_ERR_ARG, "no instance in slot %d", slot);
+    return G4S_OK;
+}
+
+int slice_len_for(const g4s_batch *b, int64_t n, int32_t requested)
+{
+    int L = requested > 0 ? requested : default_slice_len(n);
+    return L;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t g4s_batch_footprint(const g4s_spec *spec, int32_t chunk, int32_t slice_len)
+{
+    if (validate_spec(spec)) return -1;
+    const int64_t n = spec->n;
+    const int L = slice_len > 0 ? slice_len : default_slice_len(n);
+    const int64_t P = (n + L - 1) / L;
+    const int64_t words = (n + 31) / 32;
+    const int cs = chunk > 0 ? chunk : kDefaultChunk;
+    return (int64_t)layout_of(n, spec->nsyn, P).total + (int64_t)cs * (words + P) * 4 +
+           (int64_t)sizeof(DevNet) + 64;
+}
+
+int g4s_batch_create(int accum, const g4s_batch_opts *opts, g4s_batch **out)
+{
+    if (!out) return fail(G4S_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (accum != G4S_ACCUM_ATOMIC && accum != G4S_ACCUM_DET) return fail(G4S_ERR_ARG, "unknown accum %d", accum);
+    g4s_batch_opts o{};
+    if (opts) o = *opts;
+    if (o.chunk < 0 || o.chunk > 100000) return fail(G4S_ERR_ARG, "chunk %d out of range", o.chunk);
+    if (o.slice_len < 0 || o.slice_len % 64 != 0 || o.slice_len > kMaxSliceLen)
+        return fail(G4S_ERR_ARG, "slice_len must be a multiple of 64 <= %d", kMaxSliceLen);
+    if (o.reserved != 0 || o.budget < 0) return fail(G4S_ERR_ARG, "bad options");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(G4S_ERR_HIP, "no HIP device");
+    g4s_batch *b = new g4s_batch();
+    b->accum = accum;
+    b->opts = o;
+    b->chunk = o.chunk > 0 ? o.chunk : kDefaultChunk;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
+        g4s_batch_destroy(b);
+        return fail(G4S_ERR_HIP, "stream/event creation failed");
+    }
+    *out = b;
+    return G4S_OK;
+}
+
+int g4s_batch_add(g4s_batch *b, const g4s_spec *spec, int64_t steps, int keep_raster, int32_t *slot)
+{
+    if (!b || !slot) return fail(G4S_ERR_ARG, "NULL argument");
+    int rc = validate_spec(spec);
+    if (rc) return rc;
+    if (steps < 0 || steps > 0x7FFFFFFFLL) return fail(G4S_ERR_ARG, "steps out of range");
+    if (spec->nsyn > 65535) return fail(G4S_ERR_UNSUPPORTED, "batched instances need Nsyn < 65536");
+    const int64_t n = spec->n;
+    const int L = slice_len_for(b, n, b->opts.slice_len);
+    const bool det = b->accum == G4S_ACCUM_DET;
+    if (batch_lds(L, det) > 64 * 1024)
+        return fail(G4S_ERR_UNSUPPORTED, "slice length %d needs %zu bytes of LDS", L, batch_lds(L, det));
+    const int64_t P = (n + L - 1) / L;
+    const int64_t fp = g4s_batch_footprint(spec, b->chunk, L);
+    if (b->opts.budget > 0 && b->bytes_in_use + fp > b->opts.budget)
+        return fail(G4S_ERR_BUDGET, "instance needs %lld bytes, %lld of %lld in use", (long long)fp,
+                    (long long)b->bytes_in_use, (long long)b->opts.budget);
+
+    const Layout lay = layout_of(n, spec->nsyn, P);
+    void *blk = nullptr;
+    if (hipMalloc(&blk, lay.total) != hipSuccess)
+        return fail(G4S_ERR_MEM, "hipMalloc(%zu) for an instance of N=%lld", lay.total, (long long)n);
+    char *base = (char *)blk;
+    DevNet d{};
+    d.n = (int32_t)n;
+    d.n_exc = (int32_t)n_exc_of(spec);
+    d.nsyn = spec->nsyn;
+    d.words = (int32_t)((n + 31) / 32);
+    d.seed = spec->seed;
+    regime_gains(spec->regime, spec->thal_scale, &d.ge, &d.gi);
+    d.v = (float *)(base + lay.v); d.u = (float *)(base + lay.u);
+    d.a = (float *)(base + lay.a); d.b = (float *)(base + lay.b);
+    d.c = (float *)(base + lay.c); d.d = (float *)(base + lay.d);
+    d.lastI = (float *)(base + lay.lastI);
+    d.nospks = (int32_t *)(base + lay.nospks);
+    d.bm0 = (uint32_t *)(base + lay.bm0); d.bm1 = (uint32_t *)(base + lay.bm1);
+    uint2 *syn = (uint2 *)(base + lay.syn);
+    d.syn = syn;
+    d.P = (int32_t)P;
+    d.L = L;
+    {
+        const size_t ns = (size_t)n * (size_t)spec->nsyn;
+        std::vector<uint2> tab(ns);
+        build_table(spec->seed, n, d.n_exc, spec->nsyn, tab.data(), 0);
+        if (hipMemcpy(syn, tab.data(), ns * sizeof(uint2), hipMemcpyHostToDevice) != hipSuccess) {
+            hipFree(blk);
+            return fail(G4S_ERR_HIP, "upload of the synapse table failed");
+        }
+    }
+    uint16_t *soff = (uint16_t *)(base + lay.soff);
+    hipError_t e = launch_init(d, b->stream);
+    if (e == hipSuccess) e = launch_slice_offsets(d, soff, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) {
+        hipFree(blk);
+        return fail(G4S_ERR_HIP, "instance initialisation: %s", hipGetErrorString(e));
+    }
+    d.soff = soff;
+
+    int32_t s = -1;
+    for (size_t k = 0; k < b->inst.size(); ++k)
+        if (!b->inst[k].used) {
+            s = (int32_t)k;
+            break;
+        }
+    if (s < 0) {
+        s = (int32_t)b->inst.size();
+        b->inst.emplace_back();
+    }
+    Inst &in = b->inst[(size_t)s];
+    in = Inst();
+    in.used = true;
+    in.spec = *spec;
+    in.d = d;
+    in.block = blk;
+    in.bytes = fp;
+    in.steps = steps;
+    in.keep_raster = keep_raster != 0;
+    in.counts.reserve((size_t)steps);
+    if (in.keep_raster) in.raster.reserve((size_t)steps * (size_t)d.words);
+    b->bytes_in_use += fp;
+    *slot = s;
+    return G4S_OK;
+}
+
+int g4s_batch_step(g4s_batch *b, int32_t *active)
+{
+    if (!b) return fail(G4S_ERR_ARG, "batch is NULL");
+    const auto wall0 = std::chrono::steady_clock::now();
+    std::vector<int32_t> act;
+    int64_t most = 0;
+    for (size_t k = 0; k < b->inst.size(); ++k) {
+        const Inst &in = b->inst[k];
+        if (in.used && in.t < in.steps) {
+            act.push_back((int32_t)k);
+            most = std::max<int64_t>(most, in.steps - in.t);
+        }
+    }
+    if (active) *active = (int32_t)act.size();
+    b->timing = g4s_timing{};
+    if (act.empty()) return G4S_OK;
+    const int cs = (int)std::min<int64_t>(b->chunk, most);
+    const bool det = b->accum == G4S_ACCUM_DET;
+
+    // pools: instances that keep rasters first, so one copy covers them
+    std::stable_sort(act.begin(), act.end(), [&](int32_t x, int32_t y) {
+        return b->inst[(size_t)x].keep_raster > b->inst[(size_t)y].keep_raster;
+    });
+    size_t ring = 0, ring_keep = 0, bcnt = 0;
+
+    int Lmax = 64;
+    for (int32_t s : act) {
+        Inst &in = b->inst[(size_t)s];
+        in.ring_off = ring;
+        if (in.keep_raster) {
+            ring += (size_t)cs * (size_t)in.d.words;
+            ring_keep = ring;
+        }
+        in.bcnt_off = bcnt;
+        bcnt += (size_t)cs * (size_t)in.d.P;
+
+        Lmax = std::max(Lmax, (int)in.d.L);
+    }
+    int rc;
+    if ((rc = grow_dev(b, &b->d_ring, &b->h_ring, &b->ring_cap, std::max<size_t>(ring, 1)))) return rc;
+    if ((rc = grow_dev<int32_t>(b, &b->d_bcnt, nullptr, &b->bcnt_cap, bcnt))) return rc;
+    size_t dc = (size_t)b->desc_cap;
+    if ((rc = grow_dev(b, &b->d_desc, &b->h_desc, &dc, b->inst.size()))) return rc;
+    b->desc_cap = (int)dc;
+    if ((rc = grow_int(b, &b->d_slots, &b->h_slots, &b->slots_cap, (int)act.size()))) return rc;
+    if ((rc = grow_dev(b, &b->d_counts, &b->h_counts, &b->counts_cap, act.size() * (size_t)cs))) return rc;
+
+    // descriptors of this chunk
+    for (int32_t s : act) {
+        Inst &in = b->inst[(size_t)s];
+        DevNet d = in.d;
+        d.raster = in.keep_raster ? b->d_ring + in.ring_off : nullptr;
+        d.bcnt = b->d_bcnt + in.bcnt_off;
+        d.t_base = (int32_t)in.t;
+        d.rel_end = (int32_t)std::min<int64_t>(cs, in.steps - in.t);
+        d.last_t = (int32_t)(in.steps - 1);
+        b->h_desc[s] = d;
+    }
+    // work list.  Workgroup `blk` runs on XCD blk % 8.  Every instance goes
+    // whole to one XCD (its slices share that XCD's L2: the bitmap and the
+    // neighbouring segments of each row), instances are dealt to XCDs
+    // longest-first to the least loaded one (cost ~ slices * L * Nsyn), and
+    // each XCD runs its costlier slices first.
+    std::vector<int32_t> order(act);
+    auto cost = [&](int32_t s) {
+        const DevNet &d = b->inst[(size_t)s].d;
+        return (double)d.P * ((double)d.L * 8.0 + (double)d.L * (double)d.nsyn * 0.02);
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return cost(x) > cost(y); });
+    std::vector<uint2> xcd[8];
+    double load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int32_t s : order) {
+        const int x = (int)(std::min_element(load, load + 8) - load);
+        load[x] += cost(s);
+        for (int p = 0; p < b->inst[(size_t)s].d.P; ++p) xcd[x].push_back(make_uint2((uint32_t)s, (uint32_t)p));
+    }
+    size_t per = 0;
+    for (auto &v : xcd) per = std::max(per, v.size());
+    const int grid = (int)std::max<size_t>(64, (per * 8 + 63) / 64 * 64);
+    size_t wc = (size_t)b->work_cap;
+    if ((rc = grow_dev(b, &b->d_work, &b->h_work, &wc, (size_t)grid))) return rc;
+    b->work_cap = (int)wc;
+    for (int blk = 0; blk < grid; ++blk) {
+        const std::vector<uint2> &v = xcd[blk % 8];
+        const size_t q = (size_t)(blk / 8);
+        b->h_work[blk] = q < v.size() ? v[q] : make_uint2(kNoWork, 0u);
+    }
+    for (size_t k = 0; k < act.size(); ++k) b->h_slots[k] = act[k];
+    HIP_OK(hipMemcpyAsync(b->d_desc, b->h_desc, b->inst.size() * sizeof(DevNet), hipMemcpyHostToDevice, b->stream));
+    HIP_OK(hipMemcpyAsync(b->d_work, b->h_work, (size_t)grid * sizeof(uint2), hipMemcpyHostToDevice, b->stream));
+    HIP_OK(hipMemcpyAsync(b->d_slots, b->h_slots, act.size() * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
+
+    const size_t lds = batch_lds(Lmax, det);
+    const Launch L{grid, kBlock, b->stream};
+    HIP_OK(hipEventRecord(b->ev0, b->stream));
+    if (b->opts.graph < 0) {
+        for (int rel = 0; rel < cs; ++rel) HIP_OK(launch_batch_step(b->d_desc, b->d_work, grid, rel, det, lds, L));
+    } else {
+        if (!b->exec || b->g_cs != cs || b->g_grid != grid || b->g_lds != lds || b->g_gen != b->gen) {
+            drop_graph(b);
+            HIP_OK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e = hipSuccess;
+            for (int rel = 0; rel < cs && e == hipSuccess; ++rel)
+                e = launch_batch_step(b->d_desc, b->d_work, grid, rel, det, lds, L);
+            hipGraph_t g = nullptr;
+            hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+            if (e != hipSuccess || e2 != hipSuccess) {
+                if (g) hipGraphDestroy(g);
+                return fail(G4S_ERR_HIP, "batch graph capture: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+            }
+            b->graph = g;
+            HIP_OK(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+            b->g_cs = cs;
+            b->g_grid = grid;
+            b->g_lds = lds;
+            b->g_gen = b->gen;
+        }
+        HIP_OK(hipGraphLaunch(b->exec, b->stream));
+    }
+    HIP_OK(hipEventRecord(b->ev1, b->stream));
+    HIP_OK(launch_batch_counts(b->d_desc, b->d_slots, (int)act.size(), cs, b->d_counts, b->stream));
+    HIP_OK(hipMemcpyAsync(b->h_counts, b->d_counts, act.size() * (size_t)cs * sizeof(int32_t), hipMemcpyDeviceToHost,
+                          b->stream));
+    if (ring_keep)
+        HIP_OK(hipMemcpyAsync(b->h_ring, b->d_ring, ring_keep * sizeof(uint32_t), hipMemcpyDeviceToHost, b->stream));
+    HIP_OK(hipStreamSynchronize(b->stream));
+    float ms = 0.0f;
+    HIP_OK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+
+    int64_t spikes = 0, steps = 0;
+    for (size_t k = 0; k < act.size(); ++k) {
+        Inst &in = b->inst[(size_t)act[k]];
+        const int adv = (int)std::min<int64_t>(cs, in.steps - in.t);
+        for (int r = 0; r < adv; ++r) {
+            const int32_t c = b->h_counts[k * (size_t)cs + (size_t)r];
+            in.counts.push_back(c);
+            in.spikes += c;
+            spikes += c;
+        }
+        if (in.keep_raster) {
+            const uint32_t *src = b->h_ring + in.ring_off;
+            in.raster.insert(in.raster.end(), src, src + (size_t)adv * (size_t)in.d.words);
+        }
+        in.t += adv;
+        steps += adv;
+    }
+    b->timing.device_ms = ms;
+    b->timing.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+    b->timing.steps = steps;
+    b->timing.spikes = spikes;
+    b->timing.launches = cs;
+    b->timing.grid = grid;
+    return G4S_OK;
+}
+
+int g4s_batch_run(g4s_batch *b)
+{
+    if (!b) return fail(G4S_ERR_ARG, "batch is NULL");
+    g4s_timing total{};
+    for (;;) {
+        int32_t active = 0;
+        int rc = g4s_batch_step(b, &active);
+        if (rc) return rc;
+        if (active == 0) break;
+        total.device_ms += b->timing.device_ms;
+        total.wall_ms += b->timing.wall_ms;
+        total.steps += b->timing.steps;
+        total.spikes += b->timing.spikes;
+        total.launches += b->timing.launches;
+        total.grid = std::max(total.grid, b->timing.grid);
+    }
+    b->timing = total;
+    return G4S_OK;
+}
+
+int g4s_batch_info(const g4s_batch *b, int32_t slot, g4s_inst_info *info)
+{
+    int rc = check_slot(b, slot);
+    if (rc) return rc;
+    if (!info) return fail(G4S_ERR_ARG, "info is NULL");
+    const Inst &in = b->inst[(size_t)slot];
+    info->n = in.d.n;
+    info->nsyn = in.d.nsyn;
+    info->slice_len = in.d.L;
+    info->t = in.t;
+    info->steps = in.steps;
+    info->spikes = in.spikes;
+    info->bytes = in.bytes;
+    info->done = in.t >= in.steps;
+    info->slices = in.d.P;
+    return G4S_OK;
+}
+
+int g4s_batch_counts(const g4s_batch *b, int32_t slot, int32_t *counts)
+{
+    int rc = check_slot(b, slot);
+    if (rc) return rc;
+    const Inst &in = b->inst[(size_t)slot];
+    if (counts && !in.counts.empty()) memcpy(counts, in.counts.data(), in.counts.size() * sizeof(int32_t));
+    return G4S_OK;
+}
+
+int g4s_batch_raster(const g4s_batch *b, int32_t slot, uint32_t *words)
+{
+    int rc = check_slot(b, slot);
+    if (rc) return rc;
+    const Inst &in = b->inst[(size_t)slot];
+    if (!in.keep_raster) return fail(G4S_ERR_STATE, "instance %d keeps no raster", slot);
+    if (words && !in.raster.empty()) memcpy(words, in.raster.data(), in.raster.size() * sizeof(uint32_t));
+    return G4S_OK;
+}
+
+int g4s_batch_state(g4s_batch *b, int32_t slot, float *v, float *u, float *I, int32_t *nospks)
+{
+    int rc = check_slot(b, slot);
+    if (rc) return rc;
+    const DevNet &d = b->inst[(size_t)slot].d;
+    const size_t bytes = (size_t)d.n * 4;
+    HIP_OK(hipStreamSynchronize(b->stream));
+    if (v) HIP_OK(hipMemcpy(v, d.v, bytes, hipMemcpyDeviceToHost));
+    if (u) HIP_OK(hipMemcpy(u, d.u, bytes, hipMemcpyDeviceToHost));
+    if (I) HIP_OK(hipMemcpy(I, d.lastI, bytes, hipMemcpyDeviceToHost));
+    if (nospks) HIP_OK(hipMemcpy(nospks, d.nospks, bytes, hipMemcpyDeviceToHost));
+    return G4S_OK;
+}
+
+int g4s_batch_remove(g4s_batch *b, int32_t slot)
+{
+    int rc = check_slot(b, slot);
+    if (rc) return rc;
+    Inst &in = b->inst[(size_t)slot];
+    HIP_OK(hipStreamSynchronize(b->stream));
+    if (in.block) hipFree(in.block);
+    b->bytes_in_use -= in.bytes;
+    in = Inst();
+    return G4S_OK;
+}
+
+int g4s_batch_timing(const g4s_batch *b, g4s_timing *out)
+{
+    if (!b || !out) return fail(G4S_ERR_ARG, "NULL argument");
+    *out = b->timing;
+    return G4S_OK;
+}
+
+int g4s_batch_destroy(g4s_batch *b)
+{
+    if (!b) return G4S_OK;
+    if (b->stream) hipStreamSynchronize(b->stream);
+    drop_graph(b);
+    for (Inst &in : b->inst)
+        if (in.block) hipFree(in.block);
+    for (void *p : {(void *)b->d_desc, (void *)b->d_work, (void *)b->d_ring, (void *)b->d_bcnt, (void *)b->d_slots,
+                    (void *)b->d_counts})
+        if (p) hipFree(p);
+    for (void *p : {(void *)b->h_desc, (void *)b->h_work, (void *)b->h_ring, (void *)b->h_slots, (void *)b->h_counts})
+        if (p) hipHostFree(p);
+    if (b->ev0) hipEventDestroy(b->ev0);
+    if (b->ev1) hipEventDestroy(b->ev1);
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+    return G4S_OK;
+}
+
+}  // extern "C"

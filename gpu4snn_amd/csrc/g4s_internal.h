@@ -1,9 +1,22 @@
 // g4s_internal.h -- device-side view of one network and the launch entry
 // points shared between g4s_kernels.hip and g4s_runtime.hip.
 #pragma once
+#include "../../include/gpu4snn.h"
 #include "g4s_math.h"
 
 namespace g4s {
+
+// Host helpers shared by the runtime files.  fail() sets the calling
+// thread's message (g4s_last_error) and returns `code`.
+int fail(int code, const char *fmt, ...);
+int validate_spec(const g4s_spec *s);
+int64_t n_exc_of(const g4s_spec *s);
+
+#define G4S_HIP_OK(expr)                                                                           \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess) return ::g4s::fail(G4S_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
 
 constexpr uint32_t kSentinel = 0xFFFFFFFFu;   // empty spike-queue slot
 constexpr int kCtlStride = 4;                 // per-step control words
@@ -49,7 +62,15 @@ struct DevNet {
     // Per-step kernels replayed from a HIP graph read the chunk's first
     // step and the run's last step here: tinfo[0] = t0, tinfo[1] = last.
     const int32_t *tinfo;
+    // Batched instances (descriptor table in device memory): step of the
+    // chunk's first launch, launches of the chunk that advance this
+    // instance, and the instance's last step (lastI is written there).
+    int32_t t_base, rel_end, last_t, pad_;
 };
+
+// Work unit of a batched launch: slice `p` of the instance in slot `slot`;
+// slot == kNoWork pads the grid.
+constexpr uint32_t kNoWork = 0xFFFFFFFFu;
 
 struct Launch {
     int grid, block;
@@ -73,22 +94,33 @@ hipError_t launch_reduce_counts(const DevNet &d, int steps, hipStream_t s);
 hipError_t launch_push_flat(const DevNet &d, int t, int rel, const Launch &L);
 hipError_t launch_ap(const DevNet &d, int t, int rel, int last, int prev_count, const Launch &L);
 hipError_t launch_slice_offsets(const DevNet &d, uint16_t *soff, hipStream_t s);
-// AB step `rel` of the chunk; the absolute step is tinfo[0] + rel.
-hipError_t launch_slice_step(const DevNet &d, int rel, const Launch &L);
-hipError_t launch_skl_slice(const DevNet &d, int t0, int steps, int last_t, const Launch &L);
+// AB step `rel` of the chunk; the absolute step is tinfo[0] + rel.  det:
+// ordered accumulation (bit-equal to a sequential ascending push).
+hipError_t launch_slice_step(const DevNet &d, int rel, bool det, const Launch &L);
+hipError_t launch_skl_slice(const DevNet &d, int t0, int steps, int last_t, bool det, const Launch &L);
 // Co-resident workgroups per CU of the SKL slice kernel used for slice
 // length L at `block` threads.
-int skl_slice_blocks_per_cu(int L, int block);
-hipError_t launch_pull_thread(const DevNet &d, int t, int rel, int last, const Launch &L, bool cooperative);
+int skl_slice_blocks_per_cu(int L, int block, bool det);
+// Dynamic LDS bytes of the slice kernels (single network at `block`
+// threads; batched at 256).
+size_t slice_lds(int L, int block, bool det);
+size_t batch_lds(int L, bool det);
+hipError_t launch_pull_thread(const DevNet &d, int t, int rel, int last, const Launch &L);
 hipError_t launch_pull_wave(const DevNet &d, int t, int rel, int last, const Launch &L);
 hipError_t launch_ap_det(const DevNet &d, int t, int rel, int last, const Launch &L);
-hipError_t launch_skl_det(const DevNet &d, int t0, int steps, int last_t, const Launch &L);
+// Batched step `rel` of a chunk: one workgroup per work unit, grid = units
+// (padded entries exit).  block = 256.
+hipError_t launch_batch_step(const DevNet *descs, const uint2 *work, int grid, int rel, bool det, size_t lds,
+                             const Launch &L);
+// counts[k * cs + rel] = spikes of instance slots[k] at chunk step rel.
+hipError_t launch_batch_counts(const DevNet *descs, const int32_t *slots, int nslots, int cs, int32_t *counts,
+                               hipStream_t s);
 hipError_t launch_thalamic(uint64_t seed, uint32_t t, int64_t n, float *out, hipStream_t s);
 hipError_t launch_normal_words(const uint32_t *x0, const uint32_t *x1, int64_t n, float *out,
                                hipStream_t s);
 
 // Co-resident workgroups of each cooperative kernel at `block` threads.
-enum CoopKernel { kCoopAP, kCoopSKLSlice, kCoopPull, kCoopAPDet, kCoopSKLDet };
+enum CoopKernel { kCoopAP, kCoopAPDet };
 int coop_blocks_per_cu(CoopKernel k, int block, size_t shmem);
 int device_cus();
 

@@ -225,10 +225,12 @@ __global__ void __launch_bounds__(256) k_init(DevNet d)
         d.u[j] = -65.0f * p.b;
         d.nospks[j] = 0;
         d.lastI[j] = 0.0f;
-        d.isyn0[j] = 0.0f;
-        d.isyn1[j] = 0.0f;
-        d.q0[j] = kSentinel;
-        d.q1[j] = kSentinel;
+        if (d.isyn0) {  // single-network handles; batched instances have none
+            d.isyn0[j] = 0.0f;
+            d.isyn1[j] = 0.0f;
+            d.q0[j] = kSentinel;
+            d.q1[j] = kSentinel;
+        }
     }
     for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < d.words; w += stride) {
         d.bm0[w] = 0u;
@@ -443,13 +445,35 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int *s_tmp, int *tota
 
 // LDS workspace of slice_accumulate beyond acc[L]: per spike of a pass its
 // row id, first segment index and the prefix of segment lengths, plus the
-// scan scratch.  A pass handles kSpikesPerThread spikes per thread.
+// scan scratch.  A pass handles kSpikesPerThread spikes per thread.  The
+// ordered (det) variant adds, per window of kDetPerThread*B entries, a
+// count/cursor and a start per target and each entry's key and weight.
 constexpr int kSpikesPerThread = 2;
-__host__ __device__ inline size_t slice_lds_bytes(int L, int block)
+// Entry loads in flight per thread: single-network slices (up to 1024
+// threads) and batched slices (256 threads).
+constexpr int kEntriesSingle = 4;
+constexpr int kEntriesBatch = 8;
+__host__ __device__ inline size_t slice_lds_bytes(int L, int block, bool det, int ept)
 {
     const size_t cap = (size_t)kSpikesPerThread * (size_t)block;
-    return (size_t)L * 4 + cap * 4 /*ids*/ + cap * 4 /*seg start*/ + (cap + 1) * 4 /*seg prefix*/ +
-           ((size_t)block / 64 + 1) * 4;
+    size_t b = (size_t)L * 4 + cap * 4 /*ids*/ + cap * 4 /*seg start*/ + (cap + 1) * 4 /*seg prefix*/ +
+               ((size_t)block / 64 + 1) * 4;
+    if (det) b += (size_t)L * 8 /*cursor, start*/ + (size_t)ept * (size_t)block * 6 /*w, key*/;
+    return b;
+}
+
+// Entry f of the current pass (concatenated segments): binary search for
+// the last spike a with segp[a] <= f, then the table entry.
+__device__ __forceinline__ uint2 pass_entry(const DevNet &d, const uint32_t *ids, const uint32_t *seg0,
+                                            const int *segp, int m, int f)
+{
+    int a = 0, b = m - 1;
+    while (a < b) {
+        const int mid = (a + b + 1) >> 1;
+        if (segp[mid] <= f) a = mid;
+        else b = mid - 1;
+    }
+    return d.syn[(size_t)ids[a] * (size_t)d.nsyn + (size_t)seg0[a] + (size_t)(f - segp[a])];
 }
 
 // acc[0..len) <- input of slice p from the rows that spiked at step t-1.
@@ -460,7 +484,14 @@ __host__ __device__ inline size_t slice_lds_bytes(int L, int block)
 //  3. thread per synapse entry of the concatenated segments (binary search
 //     over the LDS prefix of segment lengths).
 // Spikes beyond kSpikesPerThread*B are handled in further passes.  Ends with
-// a workgroup barrier.  LDS float atomics: any order.
+// a workgroup barrier.
+// kDet = false: LDS float atomics, any order.
+// kDet = true: every acc[j] is summed in ascending presynaptic id from 0.0f,
+// bit-equal to a sequential push.  The entries of a pass are in ascending
+// (source, target) order; each window of them is counting-sorted by target
+// in LDS (count, scan, place) and each target then adds its entries in key
+// order, so the sum never depends on the order of the LDS atomics.
+template <bool kDet, int kEPT>
 __device__ __forceinline__ void slice_accumulate(const DevNet &d, int t, int p, int lo, int len, float *acc,
                                                  uint32_t *ws)
 {
@@ -471,7 +502,17 @@ __device__ __forceinline__ void slice_accumulate(const DevNet &d, int t, int p, 
     uint32_t *seg0 = ws + cap;
     int *segp = (int *)(ws + 2 * cap);
     int *scratch = (int *)(ws + 3 * cap + 1);
-    for (int i = tid; i < len; i += B) acc[i] = 0.0f;
+    // det workspace (after the scan scratch)
+    const int L = d.L;
+    int *dcur = scratch + B / kWave + 1;
+    int *dstart = dcur + L;
+    const int Wd = kEPT * B;
+    float *dw = (float *)(dstart + L);
+    uint16_t *dkey = (uint16_t *)(dw + Wd);
+    for (int i = tid; i < len; i += B) {
+        acc[i] = 0.0f;
+        if (kDet) dcur[i] = 0;
+    }
     if (t == 0) {
         __syncthreads();
         return;
@@ -493,6 +534,9 @@ __device__ __forceinline__ void slice_accumulate(const DevNet &d, int t, int p, 
     int total = 0;
     const int base = block_exclusive_scan(mine, scratch, &total);
     const size_t stride = (size_t)d.P + 1;
+    // targets owned by this thread in the ordered phases: a contiguous run
+    const int tper = (len + B - 1) / B;
+    const int tb = min(tid * tper, len), te = min(tb + tper, len);
     for (int pass = 0; pass < total; pass += cap) {
         // 1. spike ids of this pass, ascending
         if (mine > 0 && base < pass + cap && base + mine > pass) {
@@ -543,26 +587,66 @@ __device__ __forceinline__ void slice_accumulate(const DevNet &d, int t, int p, 
         }
         if (tid == 0) segp[m] = E;
         __syncthreads();
-        // 3. entries, thread per synapse; four loads in flight per thread
-        for (int f0 = tid; f0 < E; f0 += 4 * B) {
-            uint2 x[4];
+        // 3. entries, thread per synapse, windows of kEPT*B entries with all
+        // of a window's loads in flight
+        for (int f0 = 0; f0 < E; f0 += Wd) {
+            uint2 x[kEPT];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int f = f0 + k * B;
+            for (int k = 0; k < kEPT; ++k) {
+                const int f = f0 + k * B + tid;
                 x[k] = make_uint2(0xFFFFFFFFu, 0u);
-                if (f < E) {
-                    int a = 0, b = m - 1;  // last spike i with segp[i] <= f
-                    while (a < b) {
-                        const int mid = (a + b + 1) >> 1;
-                        if (segp[mid] <= f) a = mid;
-                        else b = mid - 1;
-                    }
-                    x[k] = d.syn[(size_t)ids[a] * (size_t)d.nsyn + (size_t)seg0[a] + (size_t)(f - segp[a])];
-                }
+                if (f < E) x[k] = pass_entry(d, ids, seg0, segp, m, f);
             }
+            if (!kDet) {
 #pragma unroll
-            for (int k = 0; k < 4; ++k)
-                if (x[k].x != 0xFFFFFFFFu) atomicAdd(&acc[(int)x[k].x - lo], u2f(x[k].y));
+                for (int k = 0; k < kEPT; ++k)
+                    if (x[k].x != 0xFFFFFFFFu) atomicAdd(&acc[(int)x[k].x - lo], u2f(x[k].y));
+            } else {
+                // ordered: counting sort of the window by target, then each
+                // target adds its entries in window-offset (= source) order
+#pragma unroll
+                for (int k = 0; k < kEPT; ++k)
+                    if (x[k].x != 0xFFFFFFFFu) atomicAdd(&dcur[(int)x[k].x - lo], 1);
+                __syncthreads();
+                // starts of the targets' buckets; the cursor restarts there
+                int csum = 0;
+                for (int j = tb; j < te; ++j) csum += dcur[j];
+                int nwin = 0;
+                int st = block_exclusive_scan(csum, scratch, &nwin);
+                for (int j = tb; j < te; ++j) {
+                    const int c = dcur[j];
+                    dstart[j] = st;
+                    dcur[j] = st;
+                    st += c;
+                }
+                __syncthreads();
+                // keys (window offset = ascending source) into the buckets
+#pragma unroll
+                for (int k = 0; k < kEPT; ++k)
+                    if (x[k].x != 0xFFFFFFFFu) dkey[atomicAdd(&dcur[(int)x[k].x - lo], 1)] = (uint16_t)(k * B + tid);
+                __syncthreads();
+                // each entry's rank in its bucket puts its weight in key order
+#pragma unroll
+                for (int k = 0; k < kEPT; ++k)
+                    if (x[k].x != 0xFFFFFFFFu) {
+                        const int j = (int)x[k].x - lo;
+                        const int s0 = dstart[j], s1 = dcur[j];
+                        const uint32_t key = (uint32_t)(k * B + tid);
+                        int r = 0;
+                        for (int q = s0; q < s1; ++q) r += (uint32_t)dkey[q] < key ? 1 : 0;
+                        dw[s0 + r] = u2f(x[k].y);
+                    }
+                __syncthreads();
+                // each target adds its bucket in order
+                for (int j = tb; j < te; ++j) {
+                    const int s0 = dstart[j], s1 = dcur[j];
+                    float a = acc[j];
+                    for (int q = s0; q < s1; ++q) a = a + dw[q];
+                    acc[j] = a;
+                    dcur[j] = 0;
+                }
+                __syncthreads();
+            }
         }
         __syncthreads();
     }
@@ -611,7 +695,7 @@ __global__ void __launch_bounds__(256) k_reduce_counts(DevNet d, int steps)
 
 // AB scheme: one launch per step (rel-th of the chunk; t = tinfo[0] + rel),
 // grid = P slices.  Thread state is loaded from and stored to HBM.
-template <int R>
+template <int R, bool kDet>
 __global__ void __launch_bounds__(1024) k_slice_step(DevNet d, int rel)
 {
     extern __shared__ float dyn[];
@@ -622,7 +706,7 @@ __global__ void __launch_bounds__(1024) k_slice_step(DevNet d, int rel)
     const int p = slice_of_block((int)blockIdx.x, (int)gridDim.x);
     const int lo = p * d.L;
     const int len = min(d.L, d.n - lo);
-    slice_accumulate(d, t, p, lo, len, acc, (uint32_t *)(dyn + d.L));
+    slice_accumulate<kDet, kEntriesSingle>(d, t, p, lo, len, acc, (uint32_t *)(dyn + d.L));
     const int B = (int)blockDim.x;
     const int ntile = (len + B - 1) / B;
     uint64_t masks[R];
@@ -639,11 +723,68 @@ __global__ void __launch_bounds__(1024) k_slice_step(DevNet d, int rel)
     slice_count<R>(d, rel, p, ntile, masks, &s_cnt);
 }
 
+// Batched form: one launch advances every instance of the chunk by one
+// step.  Workgroup b takes work unit work[b] = {slot, p}, the slice p of the
+// instance described by descs[slot]; the host orders the units so that each
+// XCD holds contiguous slices of the same instances.  Instances whose
+// chunk ends before step `rel` (rel >= rel_end) do nothing.
+template <bool kDet>
+__global__ void __launch_bounds__(256) k_batch_step(const DevNet *__restrict__ descs, const uint2 *__restrict__ work,
+                                                    int rel)
+{
+    extern __shared__ float dyn[];
+    __shared__ int s_cnt;
+    const uint2 wu = work[blockIdx.x];
+    if (wu.x == kNoWork) return;
+    // a private copy: the kernel's stores could otherwise alias the table,
+    // forcing a reload of every field after each store
+    const DevNet d = descs[wu.x];
+    if (rel >= d.rel_end) return;
+    const int t = d.t_base + rel;
+    const int p = (int)wu.y;
+    const int lo = p * d.L;
+    const int len = min(d.L, d.n - lo);
+    float *acc = dyn;
+    slice_accumulate<kDet, kEntriesBatch>(d, t, p, lo, len, acc, (uint32_t *)(dyn + d.L));
+    const int B = (int)blockDim.x;
+    const int tid = (int)threadIdx.x;
+    const bool last = t == d.last_t;
+    if (tid == 0) s_cnt = 0;
+    __syncthreads();
+    int c = 0;
+    for (int base = 0; base < len; base += B) {  // uniform
+        const int off = base + tid;
+        bool spk = false;
+        if (off < len) spk = neuron_step(d, t, lo + off, acc[off], last);
+        const uint64_t mask = emit_bits(d, t, rel, lo + base + (tid & ~(kWave - 1)), spk, lo + len);
+        c += __popcll(mask);
+    }
+    if (lane_id() == 0 && c) atomicAdd(&s_cnt, c);
+    __syncthreads();
+    if (tid == 0) d.bcnt[(size_t)rel * (size_t)d.P + (size_t)p] = s_cnt;
+}
+
+// Per-step spike counts of the chunk's instances: workgroup k sums the
+// slices' counts of instance slots[k] for every step of the chunk.
+__global__ void __launch_bounds__(256) k_batch_counts(const DevNet *descs, const int32_t *slots, int cs,
+                                                      int32_t *counts)
+{
+    const DevNet &d = descs[slots[blockIdx.x]];
+    for (int rel = (int)threadIdx.x; rel < cs; rel += (int)blockDim.x) {
+        int s = 0;
+        if (rel < d.rel_end) {
+            const int32_t *row = d.bcnt + (size_t)rel * (size_t)d.P;
+            for (int p = 0; p < d.P; ++p) s += row[p];
+        }
+        counts[(size_t)blockIdx.x * (size_t)cs + (size_t)rel] = s;
+    }
+}
+
 // SKL scheme: the whole chunk in one co-resident launch of P workgroups.
 // Each thread keeps the state of its R neurons (j = lo + k*B + tid) in
 // registers for the whole chunk; per step: slice accumulation from the
 // previous step's region queue, update, spike output, one grid barrier.
-template <int R>
+template <int R, bool kDet>
 __global__ void __launch_bounds__(1024) k_skl_slice(DevNet d, int t0, int steps, int last_t)
 {
     extern __shared__ float dyn[];
@@ -673,7 +814,7 @@ __global__ void __launch_bounds__(1024) k_skl_slice(DevNet d, int t0, int steps,
     uint32_t epoch = 0;
     for (int rel = 0; rel < steps; ++rel) {
         const int t = t0 + rel;
-        slice_accumulate(d, t, p, lo, len, acc, ws);
+        slice_accumulate<kDet, kEntriesSingle>(d, t, p, lo, len, acc, ws);
         uint64_t masks[R];
 #pragma unroll
         for (int k = 0; k < R; ++k) {
@@ -707,8 +848,7 @@ __global__ void __launch_bounds__(1024) k_skl_slice(DevNet d, int t0, int steps,
     }
 }
 
-// det mode, N and AB schemes: thread per neuron gathers its input and
-// updates (AB: launched cooperatively with the persistent grid).
+// det mode, N scheme: thread per neuron gathers its input and updates.
 __global__ void __launch_bounds__(256) k_pull_thread(DevNet d, int t, int rel, int last)
 {
     const int stride = gridDim.x * blockDim.x;
@@ -781,24 +921,6 @@ __global__ void __launch_bounds__(256) k_ap_det(DevNet d, int t, int rel, int la
     }
 }
 
-// det mode, SKL scheme: the whole chunk in one co-resident launch; one
-// gather+update phase and one grid barrier per step.
-__global__ void __launch_bounds__(256) k_skl_det(DevNet d, int t0, int steps, int last_t)
-{
-    uint32_t epoch = 0;
-    const int stride = gridDim.x * blockDim.x;
-    for (int rel = 0; rel < steps; ++rel) {
-        const int t = t0 + rel;
-        for (int base = blockIdx.x * blockDim.x; base < d.n; base += stride) {
-            const int j = base + (int)threadIdx.x;
-            bool spk = false;
-            if (j < d.n) spk = neuron_step(d, t, j, gather_input(d, t, j), t == last_t);
-            emit<false>(d, t, rel, base + ((int)threadIdx.x & ~(kWave - 1)), spk, j, d.n);
-        }
-        if (!grid_sync(d, epoch)) return;
-    }
-}
-
 __global__ void __launch_bounds__(256) k_thalamic(uint64_t seed, uint32_t t, int64_t n, float *out)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -839,27 +961,29 @@ hipError_t coop(const void *fn, const Launch &L, void **args, size_t shmem = 0)
     return hipLaunchCooperativeKernel(fn, dim3(L.grid), dim3(L.block), args, shmem, L.stream);
 }
 
-template <int R>
+template <int R, bool kDet>
 hipError_t launch_skl_slice_r(const DevNet &d, int t0, int steps, int last_t, const Launch &L)
 {
     DevNet dd = d;
     void *args[] = {&dd, &t0, &steps, &last_t};
-    return coop((const void *)k_skl_slice<R>, L, args, slice_lds_bytes(d.L, L.block));
+    return coop((const void *)k_skl_slice<R, kDet>, L, args, slice_lds_bytes(d.L, L.block, kDet, kEntriesSingle));
 }
 
-template <int R>
+template <int R, bool kDet>
 hipError_t launch_slice_step_r(const DevNet &d, int rel, const Launch &L)
 {
-    return plain_launch(k_slice_step<R>, L, (uint32_t)slice_lds_bytes(d.L, L.block), d, rel);
+    return plain_launch(k_slice_step<R, kDet>, L, (uint32_t)slice_lds_bytes(d.L, L.block, kDet, kEntriesSingle), d,
+                        rel);
 }
 
+template <bool kDet>
 const void *skl_slice_fn(int per_thread)
 {
-    if (per_thread <= 1) return (const void *)k_skl_slice<1>;
-    if (per_thread <= 2) return (const void *)k_skl_slice<2>;
-    if (per_thread <= 4) return (const void *)k_skl_slice<4>;
-    if (per_thread <= 8) return (const void *)k_skl_slice<8>;
-    return (const void *)k_skl_slice<16>;
+    if (per_thread <= 1) return (const void *)k_skl_slice<1, kDet>;
+    if (per_thread <= 2) return (const void *)k_skl_slice<2, kDet>;
+    if (per_thread <= 4) return (const void *)k_skl_slice<4, kDet>;
+    if (per_thread <= 8) return (const void *)k_skl_slice<8, kDet>;
+    return (const void *)k_skl_slice<16, kDet>;
 }
 
 }  // namespace
@@ -921,44 +1045,57 @@ hipError_t launch_slice_offsets(const DevNet &d, uint16_t *soff, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_slice_step(const DevNet &d, int rel, const Launch &L)
+template <bool kDet>
+hipError_t launch_slice_step_t(const DevNet &d, int rel, const Launch &L)
 {
     const int per_thread = (d.L + L.block - 1) / L.block;
-    if (per_thread <= 1) return launch_slice_step_r<1>(d, rel, L);
-    if (per_thread <= 2) return launch_slice_step_r<2>(d, rel, L);
-    if (per_thread <= 4) return launch_slice_step_r<4>(d, rel, L);
-    if (per_thread <= 8) return launch_slice_step_r<8>(d, rel, L);
-    if (per_thread <= 16) return launch_slice_step_r<16>(d, rel, L);
+    if (per_thread <= 1) return launch_slice_step_r<1, kDet>(d, rel, L);
+    if (per_thread <= 2) return launch_slice_step_r<2, kDet>(d, rel, L);
+    if (per_thread <= 4) return launch_slice_step_r<4, kDet>(d, rel, L);
+    if (per_thread <= 8) return launch_slice_step_r<8, kDet>(d, rel, L);
+    if (per_thread <= 16) return launch_slice_step_r<16, kDet>(d, rel, L);
     return hipErrorInvalidValue;
 }
 
-
-hipError_t launch_skl_slice(const DevNet &d, int t0, int steps, int last_t, const Launch &L)
+template <bool kDet>
+hipError_t launch_skl_slice_t(const DevNet &d, int t0, int steps, int last_t, const Launch &L)
 {
     const int per_thread = (d.L + L.block - 1) / L.block;
-    if (per_thread <= 1) return launch_skl_slice_r<1>(d, t0, steps, last_t, L);
-    if (per_thread <= 2) return launch_skl_slice_r<2>(d, t0, steps, last_t, L);
-    if (per_thread <= 4) return launch_skl_slice_r<4>(d, t0, steps, last_t, L);
-    if (per_thread <= 8) return launch_skl_slice_r<8>(d, t0, steps, last_t, L);
-    if (per_thread <= 16) return launch_skl_slice_r<16>(d, t0, steps, last_t, L);
+    if (per_thread <= 1) return launch_skl_slice_r<1, kDet>(d, t0, steps, last_t, L);
+    if (per_thread <= 2) return launch_skl_slice_r<2, kDet>(d, t0, steps, last_t, L);
+    if (per_thread <= 4) return launch_skl_slice_r<4, kDet>(d, t0, steps, last_t, L);
+    if (per_thread <= 8) return launch_skl_slice_r<8, kDet>(d, t0, steps, last_t, L);
+    if (per_thread <= 16) return launch_skl_slice_r<16, kDet>(d, t0, steps, last_t, L);
     return hipErrorInvalidValue;
 }
 
-int skl_slice_blocks_per_cu(int L, int block)
+hipError_t launch_slice_step(const DevNet &d, int rel, bool det, const Launch &L)
+{
+    return det ? launch_slice_step_t<true>(d, rel, L) : launch_slice_step_t<false>(d, rel, L);
+}
+
+hipError_t launch_skl_slice(const DevNet &d, int t0, int steps, int last_t, bool det, const Launch &L)
+{
+    return det ? launch_skl_slice_t<true>(d, t0, steps, last_t, L) : launch_skl_slice_t<false>(d, t0, steps, last_t, L);
+}
+
+int skl_slice_blocks_per_cu(int L, int block, bool det)
 {
     int per_cu = 0;
-    const void *fn = skl_slice_fn((L + block - 1) / block);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, slice_lds_bytes(L, block)) != hipSuccess)
+    const int r = (L + block - 1) / block;
+    const void *fn = det ? skl_slice_fn<true>(r) : skl_slice_fn<false>(r);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, slice_lds_bytes(L, block, det, kEntriesSingle)) !=
+        hipSuccess)
         return 0;
     return per_cu;
 }
 
-hipError_t launch_pull_thread(const DevNet &d, int t, int rel, int last, const Launch &L, bool cooperative)
+size_t slice_lds(int L, int block, bool det) { return slice_lds_bytes(L, block, det, kEntriesSingle); }
+size_t batch_lds(int L, bool det) { return slice_lds_bytes(L, 256, det, kEntriesBatch); }
+
+hipError_t launch_pull_thread(const DevNet &d, int t, int rel, int last, const Launch &L)
 {
-    if (!cooperative) return plain_launch(k_pull_thread, L, 0, d, t, rel, last);
-    DevNet dd = d;
-    void *args[] = {&dd, &t, &rel, &last};
-    return coop((const void *)k_pull_thread, L, args);
+    return plain_launch(k_pull_thread, L, 0, d, t, rel, last);
 }
 
 hipError_t launch_pull_wave(const DevNet &d, int t, int rel, int last, const Launch &L)
@@ -973,11 +1110,22 @@ hipError_t launch_ap_det(const DevNet &d, int t, int rel, int last, const Launch
     return coop((const void *)k_ap_det, L, args);
 }
 
-hipError_t launch_skl_det(const DevNet &d, int t0, int steps, int last_t, const Launch &L)
+
+hipError_t launch_batch_step(const DevNet *descs, const uint2 *work, int grid, int rel, bool det, size_t lds,
+                             const Launch &L)
 {
-    DevNet dd = d;
-    void *args[] = {&dd, &t0, &steps, &last_t};
-    return coop((const void *)k_skl_det, L, args);
+    Launch Lb = L;
+    Lb.grid = grid;
+    if (det) return plain_launch(k_batch_step<true>, Lb, (uint32_t)lds, descs, work, rel);
+    return plain_launch(k_batch_step<false>, Lb, (uint32_t)lds, descs, work, rel);
+}
+
+hipError_t launch_batch_counts(const DevNet *descs, const int32_t *slots, int nslots, int cs, int32_t *counts,
+                               hipStream_t s)
+{
+    if (nslots <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_batch_counts, dim3(nslots), dim3(256), 0, s, descs, slots, cs, counts);
+    return hipGetLastError();
 }
 
 hipError_t launch_thalamic(uint64_t seed, uint32_t t, int64_t n, float *out, hipStream_t s)
@@ -997,10 +1145,7 @@ int coop_blocks_per_cu(CoopKernel k, int block, size_t shmem)
     const void *fn = nullptr;
     switch (k) {
     case kCoopAP: fn = (const void *)k_ap; break;
-    case kCoopSKLSlice: fn = (const void *)k_skl_slice<16>; break;
-    case kCoopPull: fn = (const void *)k_pull_thread; break;
     case kCoopAPDet: fn = (const void *)k_ap_det; break;
-    case kCoopSKLDet: fn = (const void *)k_skl_det; break;
     }
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, shmem) != hipSuccess) return 0;

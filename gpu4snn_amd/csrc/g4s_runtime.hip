@@ -5,7 +5,6 @@
 // device->host spike copy, and the SKL single launch.  Differences by design:
 // steps run in device-resident chunks (no per-step host round trip); the
 // raster is a per-step bitmap copied once per chunk; timing uses HIP events.
-#include "../../include/gpu4snn.h"
 #include "g4s_build.h"
 #include "g4s_internal.h"
 
@@ -20,8 +19,10 @@
 using namespace g4s;
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
+
+namespace g4s {
 
 int fail(int code, const char *fmt, ...)
 {
@@ -34,11 +35,25 @@ int fail(int code, const char *fmt, ...)
     return code;
 }
 
-#define HIP_OK(expr)                                                                         \
-    do {                                                                                     \
-        hipError_t e_ = (expr);                                                              \
-        if (e_ != hipSuccess) return fail(G4S_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
-    } while (0)
+int validate_spec(const g4s_spec *s)
+{
+    if (!s) return fail(G4S_ERR_ARG, "spec is NULL");
+    if (s->n < 2 || s->n > 0x7FFFFFFFLL) return fail(G4S_ERR_ARG, "N=%lld outside [2, 2^31)", (long long)s->n);
+    if (s->nsyn < 1 || (int64_t)s->nsyn > s->n - 1)
+        return fail(G4S_ERR_ARG, "Nsyn=%d must be in [1, N-1=%lld]", s->nsyn, (long long)(s->n - 1));
+    if (s->regime < 0 || s->regime > 2) return fail(G4S_ERR_ARG, "regime=%d not in {0,1,2}", s->regime);
+    if (!(s->thal_scale >= 0.0f) || s->thal_scale > 1e6f) return fail(G4S_ERR_ARG, "thal_scale out of range");
+    if (s->n_exc > s->n) return fail(G4S_ERR_ARG, "n_exc > N");
+    return G4S_OK;
+}
+
+int64_t n_exc_of(const g4s_spec *s) { return s->n_exc >= 0 ? s->n_exc : (4 * s->n) / 5; }
+
+}  // namespace g4s
+
+#define HIP_OK G4S_HIP_OK
+
+namespace {
 
 template <class T>
 hipError_t dalloc(T **p, size_t count)
@@ -189,20 +204,6 @@ int build_det_tables(g4s_net *net)
     return G4S_OK;
 }
 
-int validate_spec(const g4s_spec *s)
-{
-    if (!s) return fail(G4S_ERR_ARG, "spec is NULL");
-    if (s->n < 2 || s->n > 0x7FFFFFFFLL) return fail(G4S_ERR_ARG, "N=%lld outside [2, 2^31)", (long long)s->n);
-    if (s->nsyn < 1 || (int64_t)s->nsyn > s->n - 1)
-        return fail(G4S_ERR_ARG, "Nsyn=%d must be in [1, N-1=%lld]", s->nsyn, (long long)(s->n - 1));
-    if (s->regime < 0 || s->regime > 2) return fail(G4S_ERR_ARG, "regime=%d not in {0,1,2}", s->regime);
-    if (!(s->thal_scale >= 0.0f) || s->thal_scale > 1e6f) return fail(G4S_ERR_ARG, "thal_scale out of range");
-    if (s->n_exc > s->n) return fail(G4S_ERR_ARG, "n_exc > N");
-    return G4S_OK;
-}
-
-int64_t n_exc_of(const g4s_spec *s) { return s->n_exc >= 0 ? s->n_exc : (4 * s->n) / 5; }
-
 Launch plain(g4s_net *net, int64_t threads, int block)
 {
     int64_t g = (threads + block - 1) / block;
@@ -249,7 +250,8 @@ int build_slices(g4s_net *net, int want)
     const int64_t L = pick_slice_len(n, want);
     const int64_t P = (n + L - 1) / L;
     if (net->slices_ready && net->d.L == L) return G4S_OK;
-    // LDS holds acc[L] plus the pass workspace (at most 64 KB per workgroup)
+    // LDS holds acc[L] plus the pass workspace (checked per run, at most
+    // 64 KB per workgroup)
     if (L * 4 > 32 * 1024)
         return fail(G4S_ERR_UNSUPPORTED, "N=%lld needs %lld slices of %lld: beyond the slice schemes' LDS", (long long)n,
                     (long long)P, (long long)L);
@@ -294,22 +296,23 @@ void drop_graph(g4s_net *net)
 // AB chunk of `cs` steps: replayed from a captured HIP graph of cs
 // launches (captured again when the chunk length or any device pointer of
 // the network changed), or launched directly.
-int run_ab_chunk(g4s_net *net, int cs, bool use_graph, int block)
+int run_ab_chunk(g4s_net *net, int cs, bool use_graph, int block, bool det)
 {
     const Launch L{net->d.P, block, net->stream};
     if (!use_graph) {
         for (int rel = 0; rel < cs; ++rel) {
             Launch Lk = L;
             HIP_OK(kev_stamp(net, &Lk));
-            HIP_OK(launch_slice_step(net->d, rel, Lk));
+            HIP_OK(launch_slice_step(net->d, rel, det, Lk));
         }
         return G4S_OK;
     }
-    if (!net->graph_exec || net->graph_steps != cs || net->graph_gen != net->gen || net->graph_block != block) {
+    const int key = block * 2 + (det ? 1 : 0);
+    if (!net->graph_exec || net->graph_steps != cs || net->graph_gen != net->gen || net->graph_block != key) {
         drop_graph(net);
         HIP_OK(hipStreamBeginCapture(net->stream, hipStreamCaptureModeThreadLocal));
         hipError_t e = hipSuccess;
-        for (int rel = 0; rel < cs && e == hipSuccess; ++rel) e = launch_slice_step(net->d, rel, L);
+        for (int rel = 0; rel < cs && e == hipSuccess; ++rel) e = launch_slice_step(net->d, rel, det, L);
         hipGraph_t g = nullptr;
         hipError_t e2 = hipStreamEndCapture(net->stream, &g);
         if (e != hipSuccess || e2 != hipSuccess) {
@@ -320,7 +323,7 @@ int run_ab_chunk(g4s_net *net, int cs, bool use_graph, int block)
         HIP_OK(hipGraphInstantiate(&net->graph_exec, g, nullptr, nullptr, 0));
         net->graph_steps = cs;
         net->graph_gen = net->gen;
-        net->graph_block = block;
+        net->graph_block = key;
     }
     HIP_OK(hipGraphLaunch(net->graph_exec, net->stream));
     return G4S_OK;
@@ -445,16 +448,14 @@ hipError_t launch_step(g4s_net *net, int alg, int accum, int block, int t, int r
         case G4S_ALG_N:
             *L = plain(net, d.n, block);
             if ((e = kev_stamp(net, L)) != hipSuccess) return e;
-            return launch_pull_thread(d, t, rel, last, *L, false);
+            return launch_pull_thread(d, t, rel, last, *L);
         case G4S_ALG_S:
             *L = plain(net, ((int64_t)d.n + 63) / 64 * 64, block);
             if ((e = kev_stamp(net, L)) != hipSuccess) return e;
             return launch_pull_wave(d, t, rel, last, *L);
-        default:  // AB
-            if ((*rc = coop_launch(net, kCoopPull, block, d.n, L))) return hipSuccess;
-            if ((e = kev_mark(net, true)) != hipSuccess) return e;
-            e = launch_pull_thread(d, t, rel, last, *L, true);
-            break;
+        default:  // AB and SKL run the slice kernels
+            *rc = fail(G4S_ERR_ARG, "scheme %d has no per-step det launch", alg);
+            return hipSuccess;
         }
         return e != hipSuccess ? e : kev_mark(net, false);
     }
@@ -495,19 +496,25 @@ int g4s_run(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, i
     const int chunk = o.chunk > 0 ? o.chunk : 1000;
     int rc;
     if ((rc = ensure_chunk(net, chunk))) return rc;
-    const bool slice = accum == G4S_ACCUM_ATOMIC && (alg == G4S_ALG_AB || alg == G4S_ALG_SKL);
-    if (accum == G4S_ACCUM_DET && (rc = build_det_tables(net))) return rc;
+    // AB and SKL run the slice kernels in both modes; det N/S/AP gather
+    // from the incoming table.
+    const bool slice = alg == G4S_ALG_AB || alg == G4S_ALG_SKL;
+    const bool det = accum == G4S_ACCUM_DET;
+    if (det && !slice && (rc = build_det_tables(net))) return rc;
     if (slice && (rc = build_slices(net, o.slices))) return rc;
     const int sblock = slice ? slice_block(net, o.block) : 0;
     if (slice && (int64_t)net->d.L > 16 * (int64_t)sblock)
         return fail(G4S_ERR_ARG, "block %d too small for slices of %d neurons (max 16 per thread)", sblock, net->d.L);
+    if (slice && slice_lds(net->d.L, sblock, det) > 64 * 1024)
+        return fail(G4S_ERR_UNSUPPORTED, "slices of %d neurons at block %d need %zu bytes of LDS (max 65536)", net->d.L,
+                    sblock, slice_lds(net->d.L, sblock, det));
 
     DevNet &d = net->d;
     Launch L{};
     if (alg == G4S_ALG_SKL && slice) {
         // every workgroup must be resident: P slices within half the
         // capacity reported for the kernel instantiation that will run
-        const int per_cu = skl_slice_blocks_per_cu(d.L, sblock);
+        const int per_cu = skl_slice_blocks_per_cu(d.L, sblock, det);
         const int cap = std::max(1, per_cu / 2) * std::max(net->cus, 1);
         if (per_cu <= 0 || cap < d.P)
             return fail(G4S_ERR_UNSUPPORTED, "SKL needs %d co-resident workgroups, device holds %d", d.P, cap);
@@ -557,17 +564,12 @@ int g4s_run(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, i
         HIP_OK(hipEventRecord(net->ev0, net->stream));
         const bool one_launch = alg == G4S_ALG_SKL;  // ev0..ev1 brackets exactly it
         if (alg == G4S_ALG_AB && slice) {
-            if ((rc = run_ab_chunk(net, cs, use_graph, sblock))) return rc;
+            if ((rc = run_ab_chunk(net, cs, use_graph, sblock, det))) return rc;
             launches += cs;
             dom_grid = d.P;
         } else if (alg == G4S_ALG_SKL) {
             hipError_t e;
-            if (slice) {
-                e = launch_skl_slice(d, t0, cs, (int)(t_end - 1), L);
-            } else {
-                if ((rc = coop_launch(net, kCoopSKLDet, block, d.n, &L))) return rc;
-                e = launch_skl_det(d, t0, cs, (int)(t_end - 1), L);
-            }
+            e = launch_skl_slice(d, t0, cs, (int)(t_end - 1), det, L);
             if (e != hipSuccess) return fail(G4S_ERR_HIP, "SKL cooperative launch (grid %d): %s", L.grid, hipGetErrorString(e));
             launches += 1;
             dom_grid = L.grid;
@@ -780,7 +782,7 @@ extern "C" int g4s_autotune(g4s_net *net, int accum, int32_t trial_steps, int fo
     if (trial_steps < 1) trial_steps = 100;
     // probe the firing rate with a default plan to find the bucket
     g4s_plan probe{};
-    probe.alg = accum == G4S_ACCUM_ATOMIC ? G4S_ALG_AB : G4S_ALG_N;
+    probe.alg = G4S_ALG_AB;
     probe.accum = accum;
     int rc = time_plan(net, &probe, std::max(20, trial_steps / 2));
     if (rc) return rc;
@@ -795,7 +797,7 @@ extern "C" int g4s_autotune(g4s_net *net, int accum, int32_t trial_steps, int fo
     }
     std::vector<g4s_plan> cands;
     for (int alg = G4S_ALG_AP; alg <= G4S_ALG_SKL; ++alg) {
-        const bool slice = accum == G4S_ACCUM_ATOMIC && (alg == G4S_ALG_AB || alg == G4S_ALG_SKL);
+        const bool slice = alg == G4S_ALG_AB || alg == G4S_ALG_SKL;
         if (slice) {
             for (int P : {64, 128, 256, 512})
                 for (int B : {256, 512, 1024}) {

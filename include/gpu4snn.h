@@ -157,6 +157,69 @@ int g4s_build_synapses_host(const g4s_spec *spec, uint32_t *post, float *w, int3
 int g4s_device_thalamic(uint64_t seed, uint32_t t, int64_t n, float *out);
 int g4s_device_normal_words(const uint32_t *x0, const uint32_t *x1, int64_t n, float *out);
 
+/* ------------------------------------------------------------------------
+ * Batched form (no reference counterpart: the reference simulates one
+ * network per process, cu:735-1332).  A batch holds many independent
+ * networks of uneven N/Nsyn resident on one device.  Each chunk advances
+ * every unfinished instance by up to `chunk` steps with ONE launch per step
+ * covering all their slices (the AB slice scheme, sliced per instance),
+ * replayed from a captured HIP graph.  Instances may be added and removed
+ * between chunks (continuous batching); an instance's results do not depend
+ * on which other instances share its launches.
+ * ---------------------------------------------------------------------- */
+typedef struct g4s_batch g4s_batch;
+
+typedef struct g4s_batch_opts {
+    int32_t chunk;       /* steps per chunk, 0 = 100                                */
+    int32_t slice_len;   /* neurons per slice (multiple of 64), 0 = per-bucket plan   */
+    int32_t graph;       /* >= 0 replays each chunk from a HIP graph, < 0 direct     */
+    int32_t reserved;    /* must be 0                                                */
+    int64_t budget;      /* device bytes the batch may hold, 0 = no limit            */
+} g4s_batch_opts;
+
+typedef struct g4s_inst_info {
+    int64_t n;           /* neurons                                                  */
+    int32_t nsyn;        /* synapses per neuron                                      */
+    int32_t slice_len;   /* neurons per slice in use                                 */
+    int64_t t;           /* steps simulated                                          */
+    int64_t steps;       /* steps requested                                          */
+    int64_t spikes;      /* spikes so far                                            */
+    int64_t bytes;       /* device bytes held                                        */
+    int32_t done;        /* t == steps                                               */
+    int32_t slices;      /* slices (workgroups per launch)                           */
+} g4s_inst_info;
+
+/* Device bytes one instance of `spec` holds in a batch with this chunk
+ * length and slice length (0 = the batch default): state, connectivity,
+ * slice offsets, raster ring and counters.  The admission rule of serving. */
+int64_t g4s_batch_footprint(const g4s_spec *spec, int32_t chunk, int32_t slice_len);
+
+int g4s_batch_create(int accum, const g4s_batch_opts *opts, g4s_batch **out);
+
+/* Build an instance (connectivity on the host, uploaded) and schedule
+ * `steps` steps.  keep_raster != 0 keeps its raster rows on the host.
+ * Returns G4S_ERR_BUDGET when it does not fit the remaining budget. */
+int g4s_batch_add(g4s_batch *b, const g4s_spec *spec, int64_t steps, int keep_raster, int32_t *slot);
+
+/* Run one chunk; *active = instances advanced (0: nothing left to do). */
+int g4s_batch_step(g4s_batch *b, int32_t *active);
+
+/* Run chunks until every instance is done. */
+int g4s_batch_run(g4s_batch *b);
+
+int g4s_batch_info(const g4s_batch *b, int32_t slot, g4s_inst_info *info);
+/* Per-step spike counts (info.t entries) and raster rows (info.t x
+ * ceil(N/32) words, keep_raster instances only) of an instance. */
+int g4s_batch_counts(const g4s_batch *b, int32_t slot, int32_t *counts);
+int g4s_batch_raster(const g4s_batch *b, int32_t slot, uint32_t *words);
+int g4s_batch_state(g4s_batch *b, int32_t slot, float *v, float *u, float *I, int32_t *nospks);
+/* Free an instance's device memory; its slot may be reused. */
+int g4s_batch_remove(g4s_batch *b, int32_t slot);
+/* Timing of the last g4s_batch_step / g4s_batch_run call (launches = kernel
+ * launches of the batched step kernel, grid = its largest grid). */
+int g4s_batch_timing(const g4s_batch *b, g4s_timing *out);
+int g4s_batch_destroy(g4s_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,152 @@
+"""GPU: the batched form (g4s_batch_*).
+
+det mode: every instance of a batch -- whatever else shares its launches,
+whenever it joined -- gives the oracle's raster and final state bit for bit.
+atomic mode: the first 100 steps equal the deterministic raster and the
+spike total stays within 1% of it (the single-network bar).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+# (N, Nsyn, seed, thalamic scale, steps): uneven sizes and lengths
+SPECS = [
+    (300, 100, 11, 1.2, 250),
+    (1000, 999, 12, 0.9, 200),
+    (2500, 1000, 13, 1.0, 300),
+    (4099, 100, 14, 1.4, 150),
+    (777, 500, 15, 1.1, 260),
+    (64, 63, 16, 1.5, 90),
+]
+
+
+def oracle_run(oracle, n, nsyn, seed, scale, steps):
+    net = oracle.Network(n, nsyn, seed, regime=1, thal_scale=scale)
+    r, c, _ = net.run(steps)
+    return net, r, c
+
+
+def check_instance(g4s, oracle, b, slot, n, nsyn, seed, scale, steps):
+    onet, oraster, ocounts = oracle_run(oracle, n, nsyn, seed, scale, steps)
+    info = b.info(slot)
+    assert info["t"] == steps and info["done"] == 1
+    words = b.raster(slot)
+    assert np.array_equal(g4s.raster_to_dense(words, n), oraster), (n, nsyn, seed)
+    assert np.array_equal(b.counts(slot), ocounts)
+    v, u, _, nos = b.state(slot)
+    assert np.array_equal(v.view(np.uint32), onet.v.view(np.uint32))
+    assert np.array_equal(u.view(np.uint32), onet.u.view(np.uint32))
+    assert np.array_equal(nos, onet.nospks)
+
+
+@pytest.mark.parametrize("chunk,graph", [(64, 0), (100, -1), (7, 0)])
+def test_batch_det_matches_oracle(g4s, oracle, chunk, graph):
+    with g4s.Batch("det", chunk=chunk, graph=graph) as b:
+        slots = [b.add(n, ns, sd, st, thal_scale=sc) for (n, ns, sd, sc, st) in SPECS]
+        b.run()
+        tm = b.timing()
+        assert tm["steps"] == sum(s[4] for s in SPECS)
+        for slot, (n, ns, sd, sc, st) in zip(slots, SPECS):
+            check_instance(g4s, oracle, b, slot, n, ns, sd, sc, st)
+
+
+@pytest.mark.parametrize("slice_len", [64, 192, 1024, 2048])
+def test_batch_det_slice_lengths(g4s, oracle, slice_len):
+    specs = [(3000, 300, 21, 1.0, 120), (5000, 1000, 22, 1.3, 100)]
+    with g4s.Batch("det", chunk=50, slice_len=slice_len) as b:
+        slots = [b.add(n, ns, sd, st, thal_scale=sc) for (n, ns, sd, sc, st) in specs]
+        b.run()
+        for slot, (n, ns, sd, sc, st) in zip(slots, specs):
+            assert b.info(slot)["slice_len"] == slice_len
+            check_instance(g4s, oracle, b, slot, n, ns, sd, sc, st)
+
+
+def test_batch_slice_length_beyond_lds(g4s):
+    """det needs more LDS per slice than atomic: 4096-neuron slices are
+    refused in det mode (G4S_ERR_UNSUPPORTED) and run in atomic mode."""
+    with g4s.Batch("det", slice_len=4096) as b:
+        with pytest.raises(g4s.G4SError) as e:
+            b.add(5000, 100, 1, 10)
+        assert e.value.code == -5
+    with g4s.Batch("atomic", slice_len=4096) as b:
+        s = b.add(5000, 100, 1, 10)
+        b.run()
+        assert b.info(s)["done"] == 1 and b.info(s)["slices"] == 2
+
+
+def test_batch_continuous_join_and_slot_reuse(g4s, oracle):
+    """Instances joining at chunk boundaries, and a freed slot reused, give
+    the same results as alone."""
+    a = (2000, 500, 31, 1.0, 300)
+    bb = (1500, 200, 32, 1.2, 120)
+    c = (900, 300, 33, 0.8, 80)
+    with g4s.Batch("det", chunk=50) as b:
+        sa = b.add(a[0], a[1], a[2], a[4], thal_scale=a[3])
+        assert b.step() == 1 and b.step() == 1
+        assert b.info(sa)["t"] == 100
+        sb = b.add(bb[0], bb[1], bb[2], bb[4], thal_scale=bb[3])
+        while b.info(sb)["done"] == 0:
+            b.step()
+        check_instance(g4s, oracle, b, sb, bb[0], bb[1], bb[2], bb[3], bb[4])
+        b.remove(sb)
+        sc = b.add(c[0], c[1], c[2], c[4], thal_scale=c[3])
+        assert sc == sb
+        b.run()
+        check_instance(g4s, oracle, b, sa, a[0], a[1], a[2], a[3], a[4])
+        check_instance(g4s, oracle, b, sc, c[0], c[1], c[2], c[3], c[4])
+        assert b.step() == 0
+
+
+def test_batch_equals_single_network(g4s):
+    """A batched instance and the same network run alone (AB det) agree."""
+    n, nsyn, seed = 6000, 700, 41
+    with g4s.Network(n, nsyn, seed, thal_scale=1.1) as net:
+        words_alone, _ = net.run(400, alg="AB", accum="det")
+        v_alone = net.state()[0]
+    with g4s.Batch("det", chunk=100) as b:
+        b.add(3000, 300, 42, 400, thal_scale=1.3)
+        s = b.add(n, nsyn, seed, 400, thal_scale=1.1)
+        b.add(700, 100, 43, 250)
+        b.run()
+        assert np.array_equal(b.raster(s), words_alone)
+        assert np.array_equal(b.state(s)[0].view(np.uint32), v_alone.view(np.uint32))
+
+
+def test_batch_atomic_first_steps_and_rates(g4s, oracle):
+    specs = [(2500, 1000, 51, 1.0, 1000), (2500, 1000, 52, 1.0, 1000), (4000, 500, 53, 1.2, 1000)]
+    with g4s.Batch("atomic", chunk=100) as b:
+        slots = [b.add(n, ns, sd, st, thal_scale=sc) for (n, ns, sd, sc, st) in specs]
+        b.run()
+        for slot, (n, ns, sd, sc, st) in zip(slots, specs):
+            _, oraster, ocounts = oracle_run(oracle, n, ns, sd, sc, st)
+            dense = g4s.raster_to_dense(b.raster(slot), n)
+            assert np.array_equal(dense[:100], oraster[:100]), (n, sd)
+            c = b.counts(slot)
+            assert abs(int(c.sum()) - int(ocounts.sum())) <= 0.01 * ocounts.sum(), (c.sum(), ocounts.sum())
+
+
+def test_batch_budget_admission(g4s):
+    fp = g4s.footprint(3000, 300, chunk=50)
+    with g4s.Batch("det", chunk=50, budget=fp + fp // 2) as b:
+        s0 = b.add(3000, 300, 61, 50)
+        with pytest.raises(g4s.G4SError) as e:
+            b.add(3000, 300, 62, 50)
+        assert e.value.code == -7
+        b.remove(s0)
+        s1 = b.add(3000, 300, 62, 50)
+        b.run()
+        assert b.info(s1)["done"] == 1
+
+
+def test_batch_counts_only_and_invalid(g4s):
+    with g4s.Batch("det", chunk=64) as b:
+        s = b.add(1000, 100, 71, 100, keep_raster=False)
+        b.run()
+        assert b.counts(s).shape == (100,)
+        with pytest.raises(g4s.G4SError):
+            b.raster(s)
+        with pytest.raises(g4s.G4SError):
+            b.info(99)
+        with pytest.raises(g4s.G4SError):
+            b.add(10, 10, 1, 5)  # Nsyn >= N
