@@ -88,7 +88,7 @@ def bench_batch(g, args):
     specs = batch_specs(args.batch, args.seed)
     total = args.warmup + args.steps
     t0 = time.perf_counter()
-    b = g.Batch(args.accum, chunk=args.chunk)
+    b = g.Batch(args.accum, chunk=args.chunk, slice_len=args.slice_len, schedule=args.schedule)
     slots = [b.add(n, ns, sd, total, thal_scale=sc, keep_raster=False) for (n, ns, sd, sc) in specs]
     print(f"batch of {len(slots)} built in {time.perf_counter() - t0:.1f} s, "
           f"{sum(n for n, _, _, _ in specs)} neurons, {sum(n * ns for n, ns, _, _ in specs) / 1e9:.2f}e9 synapses",
@@ -134,7 +134,8 @@ def bench_batch(g, args):
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded Philox networks)",
         "config": {"workload": f"{len(specs)} networks, N log-uniform [1e3, 5e4], Nsyn in {{100,500,1000}}, "
                                f"thalamic x U(0.8,1.5), {steps_timed} timed steps each",
-                   "accum": args.accum, "chunk": args.chunk, "neurons": sum(x[0] for x in specs),
+                   "accum": args.accum, "chunk": args.chunk, "slice_len": args.slice_len or "per instance",
+                   "schedule": args.schedule, "neurons": sum(x[0] for x in specs),
                    "synapses": sum(x[0] * x[1] for x in specs), "seed": args.seed},
         "events_per_s": events / wall,
         "plain": {"sim_s_per_s": p_sim / p_wall, "sample": len(idx), "scheme": "AB"},
@@ -169,6 +170,8 @@ def main():
                          "Nsyn in {100, 500, 1000}, thalamic scale U(0.8, 1.5)) in one batch, compared "
                          "with back-to-back plain runs of a sample of them")
     ap.add_argument("--chunk", type=int, default=100)
+    ap.add_argument("--slice-len", type=int, default=0, help="batched: neurons per slice, 0 = per instance")
+    ap.add_argument("--schedule", default="step", choices=["step", "queue"])
     ap.add_argument("--plain-sample", type=int, default=32)
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times with a plain copy kernel "

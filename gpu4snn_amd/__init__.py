@@ -58,7 +58,7 @@ class Timing(C.Structure):
 
 
 class BatchOpts(C.Structure):
-    _fields_ = [("chunk", C.c_int32), ("slice_len", C.c_int32), ("graph", C.c_int32), ("reserved", C.c_int32),
+    _fields_ = [("chunk", C.c_int32), ("slice_len", C.c_int32), ("graph", C.c_int32), ("schedule", C.c_int32),
                 ("budget", C.c_int64)]
 
 
@@ -264,10 +264,15 @@ def footprint(n, nsyn, chunk=0, slice_len=0, seed=0, regime=1, thal_scale=1.0):
 class Batch:
     """Many device-resident networks advanced together (g4s_batch_*)."""
 
-    def __init__(self, accum="det", chunk=0, slice_len=0, graph=0, budget=0):
+    SCHEDULES = {"step": 0, "queue": 1}
+
+    def __init__(self, accum="det", chunk=0, slice_len=0, graph=0, budget=0, schedule="step"):
+        """schedule "step": one launch per step over all instances
+        (graph-replayed unless graph < 0); "queue": one persistent launch per
+        chunk fed by a device work queue."""
         self._h = _vp()
         self.accum = accum
-        opts = BatchOpts(chunk, slice_len, graph, 0, int(budget))
+        opts = BatchOpts(chunk, slice_len, graph, self.SCHEDULES[schedule], int(budget))
         _check(lib().g4s_batch_create(ACCUM[accum], C.byref(opts), C.byref(self._h)))
 
     def close(self):

@@ -58,11 +58,14 @@ Layout layout_of(int64_t n, int32_t nsyn, int64_t P)
     return l;
 }
 
-// Default slice length: at most 512 neurons per workgroup (a multiple of 64).
-int default_slice_len(int64_t n)
+// Default slice length: 512 neurons (fewer when N is smaller).  Measured on
+// 256 cfg3-like networks: 512 beats 128, 256 and 1024, and a Nsyn-scaled
+// choice (about one 2048-entry window per slice and step) is slower -- the
+// fixed part of a workgroup (bitmap scan, offsets, update) dominates.
+int default_slice_len(int64_t n, int32_t nsyn)
 {
-    const int64_t r = (n + 63) / 64 * 64;
-    return (int)std::min<int64_t>(r, 512);
+    (void)nsyn;
+    return (int)std::min<int64_t>(512, (n + 63) / 64 * 64);
 }
 
 struct Inst {
@@ -103,6 +106,15 @@ struct g4s_batch {
     int32_t *d_counts = nullptr, *h_counts = nullptr;
     size_t counts_cap = 0;
     int slots_cap = 0;
+    // persistent schedule: unit queue, {claimed, reserved, error}, finished
+    // slices per slot
+    unsigned long long *d_units = nullptr, *h_units = nullptr;
+    size_t units_cap = 0;
+    int32_t *d_ctr = nullptr, *h_ctr = nullptr;
+    int32_t *d_done = nullptr;
+    size_t done_cap = 0;
+    int chunk_grid = 0;
+    size_t chunk_grid_lds = 0;
     // graph of one chunk
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -161,10 +173,9 @@ int check_slot(const g4s_batch *b, int32_t slot)
     return G4S_OK;
 }
 
-int slice_len_for(const g4s_batch *b, int64_t n, int32_t requested)
+int slice_len_for(int64_t n, int32_t nsyn, int32_t requested)
 {
-    int L = requested > 0 ? requested : default_slice_len(n);
-    return L;
+    return requested > 0 ? requested : default_slice_len(n, nsyn);
 }
 
 }  // namespace
@@ -175,7 +186,7 @@ int64_t g4s_batch_footprint(const g4s_spec *spec, int32_t chunk, int32_t slice_l
 {
     if (validate_spec(spec)) return -1;
     const int64_t n = spec->n;
-    const int L = slice_len > 0 ? slice_len : default_slice_len(n);
+    const int L = slice_len_for(n, spec->nsyn, slice_len);
     const int64_t P = (n + L - 1) / L;
     const int64_t words = (n + 31) / 32;
     const int cs = chunk > 0 ? chunk : kDefaultChunk;
@@ -193,7 +204,7 @@ int g4s_batch_create(int accum, const g4s_batch_opts *opts, g4s_batch **out)
     if (o.chunk < 0 || o.chunk > 100000) return fail(G4S_ERR_ARG, "chunk %d out of range", o.chunk);
     if (o.slice_len < 0 || o.slice_len % 64 != 0 || o.slice_len > kMaxSliceLen)
         return fail(G4S_ERR_ARG, "slice_len must be a multiple of 64 <= %d", kMaxSliceLen);
-    if (o.reserved != 0 || o.budget < 0) return fail(G4S_ERR_ARG, "bad options");
+    if (o.schedule < 0 || o.schedule > 1 || o.budget < 0) return fail(G4S_ERR_ARG, "bad options");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(G4S_ERR_HIP, "no HIP device");
     g4s_batch *b = new g4s_batch();
@@ -217,7 +228,7 @@ int g4s_batch_add(g4s_batch *b, const g4s_spec *spec, int64_t steps, int keep_ra
     if (steps < 0 || steps > 0x7FFFFFFFLL) return fail(G4S_ERR_ARG, "steps out of range");
     if (spec->nsyn > 65535) return fail(G4S_ERR_UNSUPPORTED, "batched instances need Nsyn < 65536");
     const int64_t n = spec->n;
-    const int L = slice_len_for(b, n, b->opts.slice_len);
+    const int L = slice_len_for(n, spec->nsyn, b->opts.slice_len);
     const bool det = b->accum == G4S_ACCUM_DET;
     if (batch_lds(L, det) > 64 * 1024)
         return fail(G4S_ERR_UNSUPPORTED, "slice length %d needs %zu bytes of LDS", L, batch_lds(L, det));
@@ -388,10 +399,50 @@ int g4s_batch_step(g4s_batch *b, int32_t *active)
 
     const size_t lds = batch_lds(Lmax, det);
     const Launch L{grid, kBlock, b->stream};
-    HIP_OK(hipEventRecord(b->ev0, b->stream));
-    if (b->opts.graph < 0) {
+    int launch_grid = grid;
+    if (b->opts.schedule == 1) {
+        // queue: the first step of every instance, costlier instances first
+        size_t total = 0;
+        for (int32_t s : act) {
+            const Inst &in = b->inst[(size_t)s];
+            total += (size_t)in.d.P * (size_t)std::min<int64_t>(cs, in.steps - in.t);
+        }
+        if ((rc = grow_dev(b, &b->d_units, &b->h_units, &b->units_cap, total))) return rc;
+        size_t dn = b->done_cap;
+        if ((rc = grow_dev<int32_t>(b, &b->d_done, nullptr, &dn, b->inst.size()))) return rc;
+        b->done_cap = dn;
+        if (!b->d_ctr) {
+            HIP_OK(hipMalloc((void **)&b->d_ctr, 4 * sizeof(int32_t)));
+            HIP_OK(hipHostMalloc((void **)&b->h_ctr, 4 * sizeof(int32_t)));
+        }
+        size_t n0 = 0;
+        for (int32_t s : order)
+            for (int p = 0; p < b->inst[(size_t)s].d.P; ++p)
+                b->h_units[n0++] = ((unsigned long long)p << 32) | (uint32_t)s;  // rel 0
+        HIP_OK(launch_fill_u32((uint32_t *)b->d_units, total * 2, 0xFFFFFFFFu, b->stream));
+        HIP_OK(launch_fill_u32((uint32_t *)b->d_done, b->inst.size(), 0u, b->stream));
+        HIP_OK(hipMemcpyAsync(b->d_units, b->h_units, n0 * sizeof(unsigned long long), hipMemcpyHostToDevice,
+                              b->stream));
+        b->h_ctr[0] = 0;
+        b->h_ctr[1] = (int32_t)n0;
+        b->h_ctr[2] = 0;
+        b->h_ctr[3] = 0;
+        HIP_OK(hipMemcpyAsync(b->d_ctr, b->h_ctr, 4 * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
+        if (b->chunk_grid == 0 || b->chunk_grid_lds != lds) {
+            const int per_cu = batch_chunk_blocks_per_cu(det, lds);
+            if (per_cu <= 0) return fail(G4S_ERR_HIP, "no occupancy for the batched chunk kernel");
+            b->chunk_grid = per_cu * device_cus();
+            b->chunk_grid_lds = lds;
+        }
+        launch_grid = (int)std::min<size_t>((size_t)b->chunk_grid, total);
+        HIP_OK(hipEventRecord(b->ev0, b->stream));
+        HIP_OK(launch_batch_chunk(b->d_desc, b->d_units, b->d_ctr, b->d_done, (int)total, det, lds,
+                                  Launch{launch_grid, kBlock, b->stream}));
+    } else if (b->opts.graph < 0) {
+        HIP_OK(hipEventRecord(b->ev0, b->stream));
         for (int rel = 0; rel < cs; ++rel) HIP_OK(launch_batch_step(b->d_desc, b->d_work, grid, rel, det, lds, L));
     } else {
+        HIP_OK(hipEventRecord(b->ev0, b->stream));
         if (!b->exec || b->g_cs != cs || b->g_grid != grid || b->g_lds != lds || b->g_gen != b->gen) {
             drop_graph(b);
             HIP_OK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -414,12 +465,16 @@ int g4s_batch_step(g4s_batch *b, int32_t *active)
         HIP_OK(hipGraphLaunch(b->exec, b->stream));
     }
     HIP_OK(hipEventRecord(b->ev1, b->stream));
+    if (b->opts.schedule == 1)
+        HIP_OK(hipMemcpyAsync(b->h_ctr, b->d_ctr, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, b->stream));
     HIP_OK(launch_batch_counts(b->d_desc, b->d_slots, (int)act.size(), cs, b->d_counts, b->stream));
     HIP_OK(hipMemcpyAsync(b->h_counts, b->d_counts, act.size() * (size_t)cs * sizeof(int32_t), hipMemcpyDeviceToHost,
                           b->stream));
     if (ring_keep)
         HIP_OK(hipMemcpyAsync(b->h_ring, b->d_ring, ring_keep * sizeof(uint32_t), hipMemcpyDeviceToHost, b->stream));
     HIP_OK(hipStreamSynchronize(b->stream));
+    if (b->opts.schedule == 1 && b->h_ctr[2] != 0)
+        return fail(G4S_ERR_TIMEOUT, "a bounded wait of the batched chunk expired");
     float ms = 0.0f;
     HIP_OK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
 
@@ -444,8 +499,8 @@ int g4s_batch_step(g4s_batch *b, int32_t *active)
     b->timing.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
     b->timing.steps = steps;
     b->timing.spikes = spikes;
-    b->timing.launches = cs;
-    b->timing.grid = grid;
+    b->timing.launches = b->opts.schedule == 1 ? 1 : cs;
+    b->timing.grid = launch_grid;
     return G4S_OK;
 }
 
@@ -547,9 +602,10 @@ int g4s_batch_destroy(g4s_batch *b)
     for (Inst &in : b->inst)
         if (in.block) hipFree(in.block);
     for (void *p : {(void *)b->d_desc, (void *)b->d_work, (void *)b->d_ring, (void *)b->d_bcnt, (void *)b->d_slots,
-                    (void *)b->d_counts})
+                    (void *)b->d_counts, (void *)b->d_units, (void *)b->d_ctr, (void *)b->d_done})
         if (p) hipFree(p);
-    for (void *p : {(void *)b->h_desc, (void *)b->h_work, (void *)b->h_ring, (void *)b->h_slots, (void *)b->h_counts})
+    for (void *p : {(void *)b->h_desc, (void *)b->h_work, (void *)b->h_ring, (void *)b->h_slots, (void *)b->h_counts,
+                    (void *)b->h_units, (void *)b->h_ctr})
         if (p) hipHostFree(p);
     if (b->ev0) hipEventDestroy(b->ev0);
     if (b->ev1) hipEventDestroy(b->ev1);

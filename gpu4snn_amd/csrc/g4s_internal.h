@@ -112,6 +112,12 @@ hipError_t launch_ap_det(const DevNet &d, int t, int rel, int last, const Launch
 // (padded entries exit).  block = 256.
 hipError_t launch_batch_step(const DevNet *descs, const uint2 *work, int grid, int rel, bool det, size_t lds,
                              const Launch &L);
+// One persistent launch running a whole chunk from the device work queue
+// `units` (total positions; ctr = {claimed, reserved, error}; done[slot] =
+// finished slices of the instance in this chunk).
+hipError_t launch_batch_chunk(const DevNet *descs, unsigned long long *units, int32_t *ctr, int32_t *done, int total,
+                              bool det, size_t lds, const Launch &L);
+int batch_chunk_blocks_per_cu(bool det, size_t lds);
 // counts[k * cs + rel] = spikes of instance slots[k] at chunk step rel.
 hipError_t launch_batch_counts(const DevNet *descs, const int32_t *slots, int nslots, int cs, int32_t *counts,
                                hipStream_t s);

@@ -445,17 +445,20 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int *s_tmp, int *tota
 
 // LDS workspace of slice_accumulate beyond acc[L]: per spike of a pass its
 // row id, first segment index and the prefix of segment lengths, plus the
-// scan scratch.  A pass handles kSpikesPerThread spikes per thread.  The
+// scan scratch.  A pass handles kSPT spikes per thread.  The
 // ordered (det) variant adds, per window of kDetPerThread*B entries, a
 // count/cursor and a start per target and each entry's key and weight.
-constexpr int kSpikesPerThread = 2;
-// Entry loads in flight per thread: single-network slices (up to 1024
-// threads) and batched slices (256 threads).
-constexpr int kEntriesSingle = 4;
+// Spikes per pass per thread and entry loads in flight per thread:
+// single-network slices (up to 1024 threads) and batched slices (256).
+constexpr int kSpikesSingle = 2, kEntriesSingle = 4;
 constexpr int kEntriesBatch = 8;
-__host__ __device__ inline size_t slice_lds_bytes(int L, int block, bool det, int ept)
+// batched det slices keep 2 spikes per thread: their LDS then allows 6
+// workgroups per CU at 512-neuron slices (4 would leave 5)
+template <bool kDet>
+constexpr int spikes_batch() { return kDet ? 2 : 4; }
+__host__ __device__ inline size_t slice_lds_bytes(int L, int block, bool det, int ept, int spt)
 {
-    const size_t cap = (size_t)kSpikesPerThread * (size_t)block;
+    const size_t cap = (size_t)spt * (size_t)block;
     size_t b = (size_t)L * 4 + cap * 4 /*ids*/ + cap * 4 /*seg start*/ + (cap + 1) * 4 /*seg prefix*/ +
                ((size_t)block / 64 + 1) * 4;
     if (det) b += (size_t)L * 8 /*cursor, start*/ + (size_t)ept * (size_t)block * 6 /*w, key*/;
@@ -483,7 +486,7 @@ __device__ __forceinline__ uint2 pass_entry(const DevNet &d, const uint32_t *ids
 //  2. thread per spike: the row's segment bounds for slice p (offset table);
 //  3. thread per synapse entry of the concatenated segments (binary search
 //     over the LDS prefix of segment lengths).
-// Spikes beyond kSpikesPerThread*B are handled in further passes.  Ends with
+// Spikes beyond kSPT*B are handled in further passes.  Ends with
 // a workgroup barrier.
 // kDet = false: LDS float atomics, any order.
 // kDet = true: every acc[j] is summed in ascending presynaptic id from 0.0f,
@@ -491,13 +494,13 @@ __device__ __forceinline__ uint2 pass_entry(const DevNet &d, const uint32_t *ids
 // (source, target) order; each window of them is counting-sorted by target
 // in LDS (count, scan, place) and each target then adds its entries in key
 // order, so the sum never depends on the order of the LDS atomics.
-template <bool kDet, int kEPT>
+template <bool kDet, int kEPT, int kSPT>
 __device__ __forceinline__ void slice_accumulate(const DevNet &d, int t, int p, int lo, int len, float *acc,
                                                  uint32_t *ws)
 {
     const int B = (int)blockDim.x;
     const int tid = (int)threadIdx.x;
-    const int cap = kSpikesPerThread * B;
+    const int cap = kSPT * B;
     uint32_t *ids = ws;
     uint32_t *seg0 = ws + cap;
     int *segp = (int *)(ws + 2 * cap);
@@ -563,11 +566,11 @@ __device__ __forceinline__ void slice_accumulate(const DevNet &d, int t, int p, 
         __syncthreads();
         const int m = min(cap, total - pass);
         // 2. segment bounds, thread per spike
-        int lens[kSpikesPerThread];
+        int lens[kSPT];
         int lsum = 0;
 #pragma unroll
-        for (int k = 0; k < kSpikesPerThread; ++k) {
-            const int i = tid * kSpikesPerThread + k;
+        for (int k = 0; k < kSPT; ++k) {
+            const int i = tid * kSPT + k;
             lens[k] = 0;
             if (i < m) {
                 const uint16_t *o = d.soff + (size_t)ids[i] * stride + (size_t)p;
@@ -580,8 +583,8 @@ __device__ __forceinline__ void slice_accumulate(const DevNet &d, int t, int p, 
         int E = 0;
         int run = block_exclusive_scan(lsum, scratch, &E);
 #pragma unroll
-        for (int k = 0; k < kSpikesPerThread; ++k) {
-            const int i = tid * kSpikesPerThread + k;
+        for (int k = 0; k < kSPT; ++k) {
+            const int i = tid * kSPT + k;
             if (i < m) segp[i] = run;
             run += lens[k];
         }
@@ -706,7 +709,7 @@ __global__ void __launch_bounds__(1024) k_slice_step(DevNet d, int rel)
     const int p = slice_of_block((int)blockIdx.x, (int)gridDim.x);
     const int lo = p * d.L;
     const int len = min(d.L, d.n - lo);
-    slice_accumulate<kDet, kEntriesSingle>(d, t, p, lo, len, acc, (uint32_t *)(dyn + d.L));
+    slice_accumulate<kDet, kEntriesSingle, kSpikesSingle>(d, t, p, lo, len, acc, (uint32_t *)(dyn + d.L));
     const int B = (int)blockDim.x;
     const int ntile = (len + B - 1) / B;
     uint64_t masks[R];
@@ -723,13 +726,40 @@ __global__ void __launch_bounds__(1024) k_slice_step(DevNet d, int rel)
     slice_count<R>(d, rel, p, ntile, masks, &s_cnt);
 }
 
-// Batched form: one launch advances every instance of the chunk by one
-// step.  Workgroup b takes work unit work[b] = {slot, p}, the slice p of the
-// instance described by descs[slot]; the host orders the units so that each
-// XCD holds contiguous slices of the same instances.  Instances whose
-// chunk ends before step `rel` (rel >= rel_end) do nothing.
+// Batched form.  A work unit is slice p of one instance at chunk step rel:
+// it accumulates that slice's input from the instance's previous step,
+// updates the slice's neurons and records its spike count.
 template <bool kDet>
-__global__ void __launch_bounds__(256) k_batch_step(const DevNet *__restrict__ descs, const uint2 *__restrict__ work,
+__device__ __forceinline__ void batch_unit(const DevNet &d, int rel, int p, float *dyn, int *s_cnt)
+{
+    const int t = d.t_base + rel;
+    const int lo = p * d.L;
+    const int len = min(d.L, d.n - lo);
+    float *acc = dyn;
+    slice_accumulate<kDet, kEntriesBatch, spikes_batch<kDet>()>(d, t, p, lo, len, acc, (uint32_t *)(dyn + d.L));
+    const int B = (int)blockDim.x;
+    const int tid = (int)threadIdx.x;
+    const bool last = t == d.last_t;
+    if (tid == 0) *s_cnt = 0;
+    __syncthreads();
+    int c = 0;
+    for (int base = 0; base < len; base += B) {  // uniform
+        const int off = base + tid;
+        bool spk = false;
+        if (off < len) spk = neuron_step(d, t, lo + off, acc[off], last);
+        const uint64_t mask = emit_bits(d, t, rel, lo + base + (tid & ~(kWave - 1)), spk, lo + len);
+        c += __popcll(mask);
+    }
+    if (lane_id() == 0 && c) atomicAdd(s_cnt, c);
+    __syncthreads();
+    if (tid == 0) d.bcnt[(size_t)rel * (size_t)d.P + (size_t)p] = *s_cnt;
+}
+
+// One launch per step: workgroup b takes work unit work[b] = {slot, p} at
+// step rel; the host orders the units so that each XCD holds whole
+// instances.  Instances whose chunk ends before rel do nothing.
+template <bool kDet>
+__global__ void __launch_bounds__(256, 6) k_batch_step(const DevNet *__restrict__ descs, const uint2 *__restrict__ work,
                                                     int rel)
 {
     extern __shared__ float dyn[];
@@ -740,28 +770,76 @@ __global__ void __launch_bounds__(256) k_batch_step(const DevNet *__restrict__ d
     // forcing a reload of every field after each store
     const DevNet d = descs[wu.x];
     if (rel >= d.rel_end) return;
-    const int t = d.t_base + rel;
-    const int p = (int)wu.y;
-    const int lo = p * d.L;
-    const int len = min(d.L, d.n - lo);
-    float *acc = dyn;
-    slice_accumulate<kDet, kEntriesBatch>(d, t, p, lo, len, acc, (uint32_t *)(dyn + d.L));
-    const int B = (int)blockDim.x;
+    batch_unit<kDet>(d, rel, (int)wu.y, dyn, &s_cnt);
+}
+
+// One persistent launch per chunk, scheduled by data flow.  units[] holds
+// every work unit of the chunk in the order it becomes ready: {slot,
+// rel << 16 | p}; the host writes the units of each instance's first step,
+// the rest of the array is kNoWork.  A workgroup claims the next position
+// (ctr[0]), waits until that unit is published, and runs it.  The workgroup
+// that completes the last slice of an instance's step (done[slot] counts
+// finished slices) publishes the instance's next step (positions reserved
+// at ctr[1]).  Instances thus advance independently; a position is only
+// waited on after every earlier position was claimed by a running
+// workgroup, and those publish all that is left, so every wait ends.  Waits
+// are bounded (ctr[2] = error flag).  Grid: any size.
+template <bool kDet>
+__global__ void __launch_bounds__(256, 6) k_batch_chunk(const DevNet *__restrict__ descs, unsigned long long *units,
+                                                     int32_t *ctr, int32_t *done, int total)
+{
+    extern __shared__ float dyn[];
+    __shared__ int s_cnt;
+    __shared__ int s_pos, s_pub, s_base;
+    __shared__ unsigned long long s_unit;
     const int tid = (int)threadIdx.x;
-    const bool last = t == d.last_t;
-    if (tid == 0) s_cnt = 0;
-    __syncthreads();
-    int c = 0;
-    for (int base = 0; base < len; base += B) {  // uniform
-        const int off = base + tid;
-        bool spk = false;
-        if (off < len) spk = neuron_step(d, t, lo + off, acc[off], last);
-        const uint64_t mask = emit_bits(d, t, rel, lo + base + (tid & ~(kWave - 1)), spk, lo + len);
-        c += __popcll(mask);
+    for (;;) {
+        if (tid == 0) {
+            int pos = atomicAdd(&ctr[0], 1);
+            unsigned long long u = 0xFFFFFFFFFFFFFFFFull;
+            if (pos < total) {
+                uint32_t spins = 0;
+                for (;;) {
+                    u = __hip_atomic_load(&units[pos], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if ((uint32_t)u != kNoWork) break;
+                    if (__hip_atomic_load(&ctr[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+                        ++spins > kSpinLimit) {
+                        __hip_atomic_store(&ctr[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        pos = total;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            }
+            s_pos = pos;
+            s_unit = u;
+        }
+        __syncthreads();
+        if (s_pos >= total) return;  // uniform
+        const unsigned long long u = s_unit;
+        const uint32_t slot = (uint32_t)u;
+        const int rel = (int)(u >> 48);
+        const int p = (int)((u >> 32) & 0xFFFFu);
+        const DevNet d = descs[slot];
+        batch_unit<kDet>(d, rel, p, dyn, &s_cnt);
+        // release this slice; the last one of the step publishes the next
+        __syncthreads();
+        if (tid == 0) {
+            const int old = __hip_atomic_fetch_add(&done[slot], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            const bool pub = old == (rel + 1) * d.P - 1 && rel + 1 < d.rel_end;
+            s_pub = pub ? 1 : 0;
+            if (pub) s_base = atomicAdd(&ctr[1], d.P);
+        }
+        __syncthreads();
+        if (s_pub) {
+            const unsigned long long hi = (unsigned long long)(rel + 1) << 48;
+            for (int q = tid; q < d.P; q += (int)blockDim.x)
+                __hip_atomic_store(&units[s_base + q], hi | ((unsigned long long)q << 32) | slot, __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
     }
-    if (lane_id() == 0 && c) atomicAdd(&s_cnt, c);
-    __syncthreads();
-    if (tid == 0) d.bcnt[(size_t)rel * (size_t)d.P + (size_t)p] = s_cnt;
 }
 
 // Per-step spike counts of the chunk's instances: workgroup k sums the
@@ -814,7 +892,7 @@ __global__ void __launch_bounds__(1024) k_skl_slice(DevNet d, int t0, int steps,
     uint32_t epoch = 0;
     for (int rel = 0; rel < steps; ++rel) {
         const int t = t0 + rel;
-        slice_accumulate<kDet, kEntriesSingle>(d, t, p, lo, len, acc, ws);
+        slice_accumulate<kDet, kEntriesSingle, kSpikesSingle>(d, t, p, lo, len, acc, ws);
         uint64_t masks[R];
 #pragma unroll
         for (int k = 0; k < R; ++k) {
@@ -966,13 +1044,13 @@ hipError_t launch_skl_slice_r(const DevNet &d, int t0, int steps, int last_t, co
 {
     DevNet dd = d;
     void *args[] = {&dd, &t0, &steps, &last_t};
-    return coop((const void *)k_skl_slice<R, kDet>, L, args, slice_lds_bytes(d.L, L.block, kDet, kEntriesSingle));
+    return coop((const void *)k_skl_slice<R, kDet>, L, args, slice_lds_bytes(d.L, L.block, kDet, kEntriesSingle, kSpikesSingle));
 }
 
 template <int R, bool kDet>
 hipError_t launch_slice_step_r(const DevNet &d, int rel, const Launch &L)
 {
-    return plain_launch(k_slice_step<R, kDet>, L, (uint32_t)slice_lds_bytes(d.L, L.block, kDet, kEntriesSingle), d,
+    return plain_launch(k_slice_step<R, kDet>, L, (uint32_t)slice_lds_bytes(d.L, L.block, kDet, kEntriesSingle, kSpikesSingle), d,
                         rel);
 }
 
@@ -1084,14 +1162,17 @@ int skl_slice_blocks_per_cu(int L, int block, bool det)
     int per_cu = 0;
     const int r = (L + block - 1) / block;
     const void *fn = det ? skl_slice_fn<true>(r) : skl_slice_fn<false>(r);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, slice_lds_bytes(L, block, det, kEntriesSingle)) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, slice_lds_bytes(L, block, det, kEntriesSingle, kSpikesSingle)) !=
         hipSuccess)
         return 0;
     return per_cu;
 }
 
-size_t slice_lds(int L, int block, bool det) { return slice_lds_bytes(L, block, det, kEntriesSingle); }
-size_t batch_lds(int L, bool det) { return slice_lds_bytes(L, 256, det, kEntriesBatch); }
+size_t slice_lds(int L, int block, bool det) { return slice_lds_bytes(L, block, det, kEntriesSingle, kSpikesSingle); }
+size_t batch_lds(int L, bool det)
+{
+    return slice_lds_bytes(L, 256, det, kEntriesBatch, det ? spikes_batch<true>() : spikes_batch<false>());
+}
 
 hipError_t launch_pull_thread(const DevNet &d, int t, int rel, int last, const Launch &L)
 {
@@ -1110,6 +1191,21 @@ hipError_t launch_ap_det(const DevNet &d, int t, int rel, int last, const Launch
     return coop((const void *)k_ap_det, L, args);
 }
 
+
+hipError_t launch_batch_chunk(const DevNet *descs, unsigned long long *units, int32_t *ctr, int32_t *done, int total,
+                              bool det, size_t lds, const Launch &L)
+{
+    if (det) return plain_launch(k_batch_chunk<true>, L, (uint32_t)lds, descs, units, ctr, done, total);
+    return plain_launch(k_batch_chunk<false>, L, (uint32_t)lds, descs, units, ctr, done, total);
+}
+
+int batch_chunk_blocks_per_cu(bool det, size_t lds)
+{
+    int per_cu = 0;
+    const void *fn = det ? (const void *)k_batch_chunk<true> : (const void *)k_batch_chunk<false>;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, lds) != hipSuccess) return 0;
+    return per_cu;
+}
 
 hipError_t launch_batch_step(const DevNet *descs, const uint2 *work, int grid, int rel, bool det, size_t lds,
                              const Launch &L)

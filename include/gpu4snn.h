@@ -57,11 +57,11 @@ typedef struct g4s_net g4s_net;
 /* Tuning of one run; zero-initialised fields select the defaults. */
 typedef struct g4s_run_opts {
     int32_t block;       /* threads per workgroup (multiple of 64; <= 256, or <= 1024
-                            for AB/SKL atomic), 0 = chosen per scheme              */
-    int32_t slices;      /* AB/SKL atomic: target slices P (workgroups), 0 = auto   */
+                            for AB/SKL), 0 = chosen per scheme                     */
+    int32_t slices;      /* AB/SKL: target slices P (workgroups), 0 = auto          */
     int32_t chunk;       /* steps per device chunk between host copies, 0 = 1000       */
-    int32_t graph;       /* AB atomic: >= 0 replays the chunk's per-step launches from
-                            a captured HIP graph, < 0 launches them directly        */
+    int32_t graph;       /* AB: >= 0 replays the chunk's per-step launches from a
+                            captured HIP graph, < 0 launches them directly          */
     int32_t kernel_events; /* != 0: bracket every launch of the dominant kernel with
                             HIP events (g4s_timing.kernel_ms); AB steps are then
                             launched directly instead of from a graph              */
@@ -161,19 +161,25 @@ int g4s_device_normal_words(const uint32_t *x0, const uint32_t *x1, int64_t n, f
  * Batched form (no reference counterpart: the reference simulates one
  * network per process, cu:735-1332).  A batch holds many independent
  * networks of uneven N/Nsyn resident on one device.  Each chunk advances
- * every unfinished instance by up to `chunk` steps with ONE launch per step
- * covering all their slices (the AB slice scheme, sliced per instance),
- * replayed from a captured HIP graph.  Instances may be added and removed
- * between chunks (continuous batching); an instance's results do not depend
- * on which other instances share its launches.
+ * every unfinished instance by up to `chunk` steps: by default ONE persistent
+ * launch per chunk whose workgroups take work units (one slice of one
+ * instance at one step, the AB slice scheme) from a device queue, each
+ * instance's next step queued as soon as its previous step completes; or one
+ * launch per step over all instances, replayed from a captured HIP graph.
+ * Instances may be added and removed between chunks (continuous batching);
+ * an instance's results do not depend on which other instances share its
+ * launches.
  * ---------------------------------------------------------------------- */
 typedef struct g4s_batch g4s_batch;
 
 typedef struct g4s_batch_opts {
     int32_t chunk;       /* steps per chunk, 0 = 100                                */
-    int32_t slice_len;   /* neurons per slice (multiple of 64), 0 = per-bucket plan   */
-    int32_t graph;       /* >= 0 replays each chunk from a HIP graph, < 0 direct     */
-    int32_t reserved;    /* must be 0                                                */
+    int32_t slice_len;   /* neurons per slice (multiple of 64), 0 = min(512, N)     */
+    int32_t graph;       /* schedule 0: >= 0 replays the chunk's launches from a HIP
+                            graph, < 0 launches them directly                       */
+    int32_t schedule;    /* 0: one launch per step over all instances;
+                            1: one persistent launch per chunk, work units queued on
+                            the device as each instance's previous step completes */
     int64_t budget;      /* device bytes the batch may hold, 0 = no limit            */
 } g4s_batch_opts;
 

@@ -40,9 +40,10 @@ def check_instance(g4s, oracle, b, slot, n, nsyn, seed, scale, steps):
     assert np.array_equal(nos, onet.nospks)
 
 
-@pytest.mark.parametrize("chunk,graph", [(64, 0), (100, -1), (7, 0)])
-def test_batch_det_matches_oracle(g4s, oracle, chunk, graph):
-    with g4s.Batch("det", chunk=chunk, graph=graph) as b:
+@pytest.mark.parametrize("chunk,graph,schedule", [(64, 0, "queue"), (7, 0, "queue"), (64, 0, "step"),
+                                                  (100, -1, "step"), (7, 0, "step")])
+def test_batch_det_matches_oracle(g4s, oracle, chunk, graph, schedule):
+    with g4s.Batch("det", chunk=chunk, graph=graph, schedule=schedule) as b:
         slots = [b.add(n, ns, sd, st, thal_scale=sc) for (n, ns, sd, sc, st) in SPECS]
         b.run()
         tm = b.timing()
@@ -98,6 +99,24 @@ def test_batch_continuous_join_and_slot_reuse(g4s, oracle):
         assert b.step() == 0
 
 
+def test_batch_queue_many_instances_det(g4s, oracle):
+    """The queue schedule with more units than resident workgroups: 40
+    instances, instances of 1 to 40 slices, chunk boundaries inside runs."""
+    rng = np.random.default_rng(5)
+    specs = []
+    for i in range(40):
+        n = int(rng.integers(200, 20000))
+        nsyn = int(min(n - 1, rng.choice([50, 200, 400])))
+        specs.append((n, nsyn, 1000 + i, float(rng.uniform(0.8, 1.5)), int(rng.integers(30, 130))))
+    with g4s.Batch("det", chunk=50, slice_len=512) as b:
+        slots = [b.add(n, ns, sd, st, thal_scale=sc) for (n, ns, sd, sc, st) in specs]
+        b.run()
+        for slot, (n, ns, sd, sc, st) in list(zip(slots, specs))[::4]:
+            check_instance(g4s, oracle, b, slot, n, ns, sd, sc, st)
+        for slot, (n, ns, sd, sc, st) in zip(slots, specs):
+            assert b.info(slot)["done"] == 1
+
+
 def test_batch_equals_single_network(g4s):
     """A batched instance and the same network run alone (AB det) agree."""
     n, nsyn, seed = 6000, 700, 41
@@ -113,9 +132,10 @@ def test_batch_equals_single_network(g4s):
         assert np.array_equal(b.state(s)[0].view(np.uint32), v_alone.view(np.uint32))
 
 
-def test_batch_atomic_first_steps_and_rates(g4s, oracle):
+@pytest.mark.parametrize("schedule", ["queue", "step"])
+def test_batch_atomic_first_steps_and_rates(g4s, oracle, schedule):
     specs = [(2500, 1000, 51, 1.0, 1000), (2500, 1000, 52, 1.0, 1000), (4000, 500, 53, 1.2, 1000)]
-    with g4s.Batch("atomic", chunk=100) as b:
+    with g4s.Batch("atomic", chunk=100, schedule=schedule) as b:
         slots = [b.add(n, ns, sd, st, thal_scale=sc) for (n, ns, sd, sc, st) in specs]
         b.run()
         for slot, (n, ns, sd, sc, st) in zip(slots, specs):
