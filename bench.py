@@ -147,6 +147,73 @@ def bench_batch(g, args):
     }
 
 
+def bench_serve(g, args):
+    """Open-loop serving: Poisson arrivals for args.serve seconds."""
+    import numpy as np
+    rng = np.random.default_rng(args.seed + 1)
+    t_arr, t = [], 0.0
+    while True:
+        t += float(rng.exponential(1.0 / args.rate))
+        if t >= args.serve:
+            break
+        t_arr.append(t)
+    specs = batch_specs(len(t_arr), args.seed + 2)
+    steps = [int(x) for x in rng.integers(100, 5001, len(t_arr))]
+    srv = g.Server(args.accum, chunk=args.chunk, budget=int(args.budget_gb * (1 << 30)))
+    ids = []
+    t0 = time.perf_counter()
+    last_report = t0
+    for k, ta in enumerate(t_arr):
+        dt = ta - (time.perf_counter() - t0)
+        if dt > 0:
+            time.sleep(dt)
+        n, ns, sd, sc = specs[k]
+        ids.append(srv.submit(n, ns, sd, steps[k], thal_scale=sc, keep_raster=False))
+        if time.perf_counter() - last_report > 10:
+            st = srv.stats()
+            print(f"t={time.perf_counter() - t0:.0f}s submitted {st['submitted']} done {st['completed']} "
+                  f"active {st['active']} pending {st['pending']}", file=sys.stderr, flush=True)
+            last_report = time.perf_counter()
+    srv.drain(600)
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    infos = [srv.info(r) for r in ids]
+    lat = np.array([i["done_s"] - i["submit_s"] for i in infos])
+    first = min(i["submit_s"] for i in infos)
+    last = max(i["done_s"] for i in infos)
+    sim_s = sum(steps) / 1000.0
+    events = sum(i["spikes"] * specs[k][1] for k, i in enumerate(infos))
+    srv.close()
+    # back-to-back plain runs of a sample of the same requests
+    idx = list(range(0, len(ids), max(1, len(ids) // max(1, args.plain_sample))))[:args.plain_sample]
+    p_wall = p_sim = 0.0
+    for k in idx:
+        n, ns, sd, sc = specs[k]
+        t1 = time.perf_counter()
+        with g.Network(n, ns, sd, thal_scale=sc) as net:
+            net.run(steps[k], alg="AB", accum=args.accum, raster=False)
+        p_wall += time.perf_counter() - t1
+        p_sim += steps[k] / 1000.0
+    agg = sim_s / (last - first)
+    return {
+        "metric": "served: p50/p99 request completion latency and aggregate simulated seconds per second",
+        "value": agg, "unit": "sim-s/s", "n_gpus": 1, "higher_is_better": True, "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded Philox networks, Poisson arrivals)",
+        "config": {"workload": f"open-loop Poisson {args.rate} req/s for {args.serve} s, N log-uniform [1e3, 5e4], "
+                               f"Nsyn in {{100,500,1000}}, thalamic x U(0.8,1.5), steps U[100,5000]",
+                   "accum": args.accum, "chunk": args.chunk, "budget_gb": args.budget_gb,
+                   "requests": len(ids), "offered_sim_s_per_s": sim_s / args.serve, "seed": args.seed},
+        "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                      "mean": float(lat.mean()), "max": float(lat.max())},
+        "events_per_s": events / (last - first), "wall_s": wall,
+        "server": {k: st[k] for k in ("completed", "failed", "chunks", "max_active", "max_bytes", "busy_ms",
+                                      "device_ms")},
+        "plain": {"sim_s_per_s": p_sim / p_wall, "sample": len(idx), "scheme": "AB",
+                  "note": "each request alone: create (device connectivity build) + run, back to back"},
+        "speedup_vs_plain": agg / (p_sim / p_wall),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,6 +236,11 @@ def main():
                     help="batched form instead: COUNT independent networks (N log-uniform in [1e3, 5e4], "
                          "Nsyn in {100, 500, 1000}, thalamic scale U(0.8, 1.5)) in one batch, compared "
                          "with back-to-back plain runs of a sample of them")
+    ap.add_argument("--serve", type=float, default=0.0, metavar="SECONDS",
+                    help="served form instead: open-loop Poisson arrivals at --rate req/s for SECONDS, each "
+                         "a batched-form network with steps U[100, 5000]; p50/p99 latency, aggregate sim-s/s")
+    ap.add_argument("--rate", type=float, default=50.0)
+    ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--chunk", type=int, default=100)
     ap.add_argument("--slice-len", type=int, default=0, help="batched: neurons per slice, 0 = per instance")
     ap.add_argument("--schedule", default="step", choices=["step", "queue"])
@@ -199,6 +271,9 @@ def main():
 
     if args.batch > 0:
         print(json.dumps(bench_batch(g, args)), flush=True)
+        return
+    if args.serve > 0:
+        print(json.dumps(bench_serve(g, args)), flush=True)
         return
 
     t_build = time.perf_counter()

@@ -11,8 +11,8 @@ import os
 
 import numpy as np
 
-__all__ = ["ALG", "ACCUM", "Network", "Batch", "lib", "G4SError", "words_of", "raster_to_dense", "n_exc_of",
-           "footprint"]
+__all__ = ["ALG", "ACCUM", "Network", "Batch", "Server", "lib", "G4SError", "words_of", "raster_to_dense",
+           "n_exc_of", "footprint"]
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "libgpu4snn.so")
@@ -68,6 +68,23 @@ class InstInfo(C.Structure):
                 ("slices", C.c_int32)]
 
 
+class ServerOpts(C.Structure):
+    _fields_ = [("accum", C.c_int32), ("chunk", C.c_int32), ("slice_len", C.c_int32), ("reserved", C.c_int32),
+                ("budget", C.c_int64)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("state", C.c_int32), ("error", C.c_int32), ("n", C.c_int64), ("steps", C.c_int64),
+                ("t", C.c_int64), ("spikes", C.c_int64), ("submit_s", C.c_double), ("admit_s", C.c_double),
+                ("done_s", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(k, C.c_int64) for k in ("submitted", "completed", "failed", "pending", "active", "chunks", "steps",
+                                          "sim_steps_done", "bytes_in_use", "max_bytes", "max_active")] + \
+               [(k, C.c_double) for k in ("busy_ms", "device_ms", "now_s")]
+
+
 _vp = C.c_void_p
 
 
@@ -93,6 +110,7 @@ def lib():
             "g4s_last_error": ([], C.c_char_p),
             "g4s_version": ([], C.c_int),
             "g4s_build_synapses_host": ([C.POINTER(Spec), _vp, _vp, C.c_int32], C.c_int),
+            "g4s_build_synapses_device": ([C.POINTER(Spec), _vp, _vp], C.c_int),
             "g4s_device_thalamic": ([C.c_uint64, C.c_uint32, C.c_int64, _vp], C.c_int),
             "g4s_device_normal_words": ([_vp, _vp, C.c_int64, _vp], C.c_int),
             "g4s_autotune": ([_vp, C.c_int, C.c_int32, C.c_int, C.POINTER(Plan)], C.c_int),
@@ -108,6 +126,16 @@ def lib():
             "g4s_batch_remove": ([_vp, C.c_int32], C.c_int),
             "g4s_batch_timing": ([_vp, C.POINTER(Timing)], C.c_int),
             "g4s_batch_destroy": ([_vp], C.c_int),
+            "g4s_server_create": ([C.POINTER(ServerOpts), C.POINTER(_vp)], C.c_int),
+            "g4s_server_submit": ([_vp, C.POINTER(Spec), C.c_int64, C.c_int, C.POINTER(C.c_int64)], C.c_int),
+            "g4s_server_info": ([_vp, C.c_int64, C.POINTER(RequestInfo)], C.c_int),
+            "g4s_server_wait": ([_vp, C.c_int64, C.c_double], C.c_int),
+            "g4s_server_drain": ([_vp, C.c_double], C.c_int),
+            "g4s_server_counts": ([_vp, C.c_int64, _vp], C.c_int),
+            "g4s_server_raster": ([_vp, C.c_int64, _vp], C.c_int),
+            "g4s_server_release": ([_vp, C.c_int64], C.c_int),
+            "g4s_server_get_stats": ([_vp, C.POINTER(ServerStats)], C.c_int),
+            "g4s_server_destroy": ([_vp], C.c_int),
         }
         for name, (args, res) in sigs.items():
             f = getattr(L, name)
@@ -153,6 +181,15 @@ def build_synapses_host(n, nsyn, seed, regime=1, thal_scale=1.0, threads=0):
     post = np.empty(int(n) * int(nsyn), np.uint32)
     w = np.empty(int(n) * int(nsyn), np.float32)
     _check(lib().g4s_build_synapses_host(C.byref(spec), _ptr(post), _ptr(w), threads))
+    return post, w
+
+
+def build_synapses_device(n, nsyn, seed, regime=1, thal_scale=1.0):
+    """Connectivity table built by the device builder and copied back."""
+    spec = make_spec(n, nsyn, seed, regime, thal_scale)
+    post = np.empty(int(n) * int(nsyn), np.uint32)
+    w = np.empty(int(n) * int(nsyn), np.float32)
+    _check(lib().g4s_build_synapses_device(C.byref(spec), _ptr(post), _ptr(w)))
     return post, w
 
 
@@ -338,3 +375,71 @@ class Batch:
         t = Timing()
         _check(lib().g4s_batch_timing(self._h, C.byref(t)))
         return {k: getattr(t, k) for k, _ in Timing._fields_}
+
+
+REQ_STATES = {0: "pending", 1: "active", 2: "done", 3: "failed"}
+
+
+class Server:
+    """Serving loop over a batch (g4s_server_*): submit requests from any
+    thread; a worker admits them by device footprint and runs them together."""
+
+    def __init__(self, accum="det", chunk=0, budget=0, slice_len=0):
+        self._h = _vp()
+        opts = ServerOpts(ACCUM[accum], chunk, slice_len, 0, int(budget))
+        _check(lib().g4s_server_create(C.byref(opts), C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().g4s_server_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, n, nsyn, seed, steps, regime=1, thal_scale=1.0, keep_raster=True):
+        rid = C.c_int64()
+        spec = make_spec(n, nsyn, seed, regime, thal_scale)
+        _check(lib().g4s_server_submit(self._h, C.byref(spec), int(steps), int(bool(keep_raster)), C.byref(rid)))
+        return rid.value
+
+    def info(self, rid):
+        i = RequestInfo()
+        _check(lib().g4s_server_info(self._h, rid, C.byref(i)))
+        d = {k: getattr(i, k) for k, _ in RequestInfo._fields_}
+        d["state"] = REQ_STATES[d["state"]]
+        return d
+
+    def wait(self, rid, timeout=0.0):
+        _check(lib().g4s_server_wait(self._h, rid, float(timeout)))
+
+    def drain(self, timeout=0.0):
+        _check(lib().g4s_server_drain(self._h, float(timeout)))
+
+    def counts(self, rid):
+        c = np.zeros(self.info(rid)["t"], np.int32)
+        _check(lib().g4s_server_counts(self._h, rid, _ptr(c)))
+        return c
+
+    def raster(self, rid):
+        i = self.info(rid)
+        w = np.zeros((i["t"], words_of(i["n"])), np.uint32)
+        _check(lib().g4s_server_raster(self._h, rid, _ptr(w)))
+        return w
+
+    def release(self, rid):
+        _check(lib().g4s_server_release(self._h, rid))
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().g4s_server_get_stats(self._h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}

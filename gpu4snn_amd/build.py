@@ -17,7 +17,7 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libgpu4snn.so")
 CLI = os.path.join(PKG, "bin", "gpu4snn")
 
-LIB_SOURCES = ["g4s_kernels.hip", "g4s_build.hip", "g4s_runtime.hip", "g4s_batch.hip"]
+LIB_SOURCES = ["g4s_kernels.hip", "g4s_build.hip", "g4s_runtime.hip", "g4s_batch.hip", "g4s_serve.hip"]
 CLI_SOURCES = ["g4s_cli.cpp"]
 
 # -ffp-contract=off: neuron arithmetic must equal the host's binary32 ops.

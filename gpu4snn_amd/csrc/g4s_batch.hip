@@ -260,14 +260,9 @@ int g4s_batch_add(g4s_batch *b, const g4s_spec *spec, int64_t steps, int keep_ra
     d.syn = syn;
     d.P = (int32_t)P;
     d.L = L;
-    {
-        const size_t ns = (size_t)n * (size_t)spec->nsyn;
-        std::vector<uint2> tab(ns);
-        build_table(spec->seed, n, d.n_exc, spec->nsyn, tab.data(), 0);
-        if (hipMemcpy(syn, tab.data(), ns * sizeof(uint2), hipMemcpyHostToDevice) != hipSuccess) {
-            hipFree(blk);
-            return fail(G4S_ERR_HIP, "upload of the synapse table failed");
-        }
+    if ((rc = build_synapses(spec, syn, b->stream))) {
+        hipFree(blk);
+        return rc;
     }
     uint16_t *soff = (uint16_t *)(base + lay.soff);
     hipError_t e = launch_init(d, b->stream);

@@ -11,6 +11,9 @@ namespace g4s {
 int fail(int code, const char *fmt, ...);
 int validate_spec(const g4s_spec *s);
 int64_t n_exc_of(const g4s_spec *s);
+// Outgoing table of `spec` into device memory `syn`: built on the device
+// (Nsyn <= kDeviceBuildMaxNsyn) or on the host and uploaded; same table.
+int build_synapses(const g4s_spec *spec, uint2 *syn, hipStream_t stream);
 
 #define G4S_HIP_OK(expr)                                                                           \
     do {                                                                                           \
@@ -121,6 +124,11 @@ int batch_chunk_blocks_per_cu(bool det, size_t lds);
 // counts[k * cs + rel] = spikes of instance slots[k] at chunk step rel.
 hipError_t launch_batch_counts(const DevNet *descs, const int32_t *slots, int nslots, int cs, int32_t *counts,
                                hipStream_t s);
+// Rows [row0, row0+rows) of the outgoing table built on the device, equal
+// to build_table's; tab points at row row0.  Nsyn <= 1024.
+constexpr int kDeviceBuildMaxNsyn = 1024;
+hipError_t launch_build_rows(uint64_t seed, int64_t n, int64_t n_exc, int32_t nsyn, float g_exc, float g_inh, uint2 *tab,
+                             int64_t row0, int64_t rows, hipStream_t s);
 hipError_t launch_thalamic(uint64_t seed, uint32_t t, int64_t n, float *out, hipStream_t s);
 hipError_t launch_normal_words(const uint32_t *x0, const uint32_t *x1, int64_t n, float *out,
                                hipStream_t s);

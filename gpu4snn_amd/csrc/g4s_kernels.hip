@@ -999,6 +999,163 @@ __global__ void __launch_bounds__(256) k_ap_det(DevNet d, int t, int rel, int la
     }
 }
 
+// ---------------------------------------------------------------------------
+// Connectivity on the device, bit-equal to the host builder (build_table):
+// row i holds the values of Floyd's algorithm over the m = N-1 non-self
+// candidates -- for d = 0..Nsyn-1 with jj_d = m-Nsyn+d: t_d uniform on
+// [0, jj_d] from Philox (i, d, CONN); take t_d unless already taken, else
+// jj_d -- mapped v -> v + (v >= i), sorted, with weights g*U(Philox(i, j,
+// WEIGHT)).  Restated for a wave per row:
+//  * a value below J0 = jj_0 can only be taken as a draw, so draw d takes
+//    t_d < J0 exactly when d is its first occurrence (found by sorting the
+//    pairs (t_d, d));
+//  * a value jj_b >= J0 is taken either as draw b's fallback (bit b) or as
+//    a draw t_k = jj_b at k >= b; only draws t_k >= J0 read that state, so
+//    one lane resolves those draws in order over an Nsyn-bit set -- in
+//    sparse rows a handful per row (about Nsyn^2 / 2N).
+// One 64-thread workgroup per row; Nsyn <= kDevRowMax.
+// ---------------------------------------------------------------------------
+constexpr int kDevRowMax = 1024;
+
+__device__ __forceinline__ void sort64_wave(uint64_t *k, int S)
+{
+    for (int kk = 2; kk <= S; kk <<= 1)
+        for (int j = kk >> 1; j > 0; j >>= 1) {
+            for (int i = (int)threadIdx.x; i < S; i += 64) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const uint64_t a = k[i], b = k[l];
+                    const bool up = (i & kk) == 0;
+                    if ((a > b) == up) {
+                        k[i] = b;
+                        k[l] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+}
+
+__device__ __forceinline__ void sort32_wave(uint32_t *k, int S)
+{
+    for (int kk = 2; kk <= S; kk <<= 1)
+        for (int j = kk >> 1; j > 0; j >>= 1) {
+            for (int i = (int)threadIdx.x; i < S; i += 64) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const uint32_t a = k[i], b = k[l];
+                    const bool up = (i & kk) == 0;
+                    if ((a > b) == up) {
+                        k[i] = b;
+                        k[l] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+}
+
+__global__ void __launch_bounds__(64) k_build_rows(uint64_t seed, int64_t n, int64_t n_exc, int32_t nsyn, float g_exc,
+                                                   float g_inh, uint2 *tab, int64_t row0, int64_t rows)
+{
+    __shared__ uint64_t key[kDevRowMax];
+    __shared__ uint32_t tv[kDevRowMax];
+    __shared__ uint32_t val[kDevRowMax];
+    __shared__ uint8_t first[kDevRowMax];
+    __shared__ uint16_t hi_list[kDevRowMax];
+    __shared__ uint32_t taken[kDevRowMax / 32];
+    __shared__ int s_nhi;
+    const int lane = (int)threadIdx.x;
+    const uint64_t m = (uint64_t)(n - 1);
+    const uint64_t J0 = m - (uint64_t)nsyn;
+    int S = 64;
+    while (S < nsyn) S <<= 1;
+    for (int64_t i = row0 + blockIdx.x; i < row0 + rows; i += gridDim.x) {
+        // 1. draws
+        for (int d = lane; d < S; d += 64) {
+            if (d < nsyn) {
+                const uint64_t jj = J0 + (uint64_t)d;
+                const Word4 o = draw(seed, (uint32_t)i, (uint32_t)d, kTagConn);
+                const uint64_t r = ((uint64_t)o.x1 << 32) | o.x0;
+                const uint32_t t = (uint32_t)__umul64hi(r, jj + 1u);
+                tv[d] = t;
+                key[d] = ((uint64_t)t << 32) | (uint32_t)d;
+            } else {
+                key[d] = ~0ull;
+            }
+        }
+        for (int w = lane; w < kDevRowMax / 32; w += 64) taken[w] = 0u;
+        __syncthreads();
+        // 2. first occurrences of each value
+        sort64_wave(key, S);
+        for (int q = lane; q < nsyn; q += 64) {
+            const uint64_t kq = key[q];
+            const bool f = q == 0 || (key[q - 1] >> 32) != (kq >> 32);
+            first[(uint32_t)kq] = f ? 1 : 0;
+        }
+        __syncthreads();
+        // 3. draws below J0 settle at once; fallbacks of repeated ones take
+        //    their jj; the others are listed in ascending d
+        int nhi = 0;
+        for (int base = 0; base < S; base += 64) {
+            const int d = base + lane;
+            bool hi = false;
+            if (d < nsyn) {
+                const uint32_t t = tv[d];
+                if ((uint64_t)t < J0) {
+                    if (first[d]) {
+                        val[d] = t;
+                    } else {
+                        val[d] = (uint32_t)(J0 + (uint64_t)d);
+                        atomicOr(&taken[d >> 5], 1u << (d & 31));
+                    }
+                } else {
+                    hi = true;
+                }
+            }
+            const uint64_t mask = __ballot(hi);
+            if (hi) hi_list[nhi + bits_below(mask)] = (uint16_t)d;
+            nhi += __popcll(mask);
+        }
+        __syncthreads();
+        // 4. draws at or above J0, in order, by one lane
+        if (lane == 0) {
+            for (int h = 0; h < nhi; ++h) {
+                const int d = hi_list[h];
+                const uint32_t b = (uint32_t)((uint64_t)tv[d] - J0);
+                const bool is_taken = (int)b == d || ((taken[b >> 5] >> (b & 31)) & 1u);
+                if (is_taken) {
+                    val[d] = (uint32_t)(J0 + (uint64_t)d);
+                    taken[d >> 5] |= 1u << (d & 31);
+                } else {
+                    val[d] = tv[d];
+                    taken[b >> 5] |= 1u << (b & 31);
+                }
+            }
+        }
+        __syncthreads();
+        // 5. skip the presynaptic neuron itself, sort, weights
+        for (int d = lane; d < S; d += 64) {
+            if (d < nsyn) {
+                const uint32_t v = val[d];
+                val[d] = v + ((int64_t)v >= i ? 1u : 0u);
+            } else {
+                val[d] = 0xFFFFFFFFu;
+            }
+        }
+        __syncthreads();
+        sort32_wave(val, S);
+        const float g = i < n_exc ? g_exc : g_inh;
+        uint2 *out = tab + (size_t)(i - row0) * (size_t)nsyn;
+        for (int d = lane; d < nsyn; d += 64) {
+            const uint32_t j = val[d];
+            const Word4 o = draw(seed, (uint32_t)i, j, kTagWeight);
+            out[d] = make_uint2(j, f2u(g * u01(o.x0)));
+        }
+        __syncthreads();
+    }
+}
+
 __global__ void __launch_bounds__(256) k_thalamic(uint64_t seed, uint32_t t, int64_t n, float *out)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -1221,6 +1378,15 @@ hipError_t launch_batch_counts(const DevNet *descs, const int32_t *slots, int ns
 {
     if (nslots <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_batch_counts, dim3(nslots), dim3(256), 0, s, descs, slots, cs, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_rows(uint64_t seed, int64_t n, int64_t n_exc, int32_t nsyn, float g_exc, float g_inh, uint2 *tab,
+                             int64_t row0, int64_t rows, hipStream_t s)
+{
+    if (nsyn < 1 || nsyn > kDevRowMax || rows <= 0) return hipErrorInvalidValue;
+    const int grid = (int)std::min<int64_t>(rows, 65536);
+    hipLaunchKernelGGL(k_build_rows, dim3(grid), dim3(64), 0, s, seed, n, n_exc, nsyn, g_exc, g_inh, tab, row0, rows);
     return hipGetLastError();
 }
 

@@ -49,6 +49,26 @@ int validate_spec(const g4s_spec *s)
 
 int64_t n_exc_of(const g4s_spec *s) { return s->n_exc >= 0 ? s->n_exc : (4 * s->n) / 5; }
 
+int build_synapses(const g4s_spec *spec, uint2 *syn, hipStream_t stream)
+{
+    const int64_t n = spec->n;
+    const int32_t nsyn = spec->nsyn;
+    const int64_t n_exc = n_exc_of(spec);
+    if (nsyn <= kDeviceBuildMaxNsyn) {
+        const float scale = 1000.0f / (float)nsyn;  // the host builder's gains
+        hipError_t e = launch_build_rows(spec->seed, n, n_exc, nsyn, scale * 0.5f, -scale, syn, 0, n, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) return fail(G4S_ERR_HIP, "device connectivity build: %s", hipGetErrorString(e));
+        return G4S_OK;
+    }
+    const size_t ns = (size_t)n * (size_t)nsyn;
+    std::vector<uint2> tab(ns);
+    build_table(spec->seed, n, n_exc, nsyn, tab.data(), 0);
+    if (hipMemcpy(syn, tab.data(), ns * sizeof(uint2), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(G4S_ERR_HIP, "upload of the synapse table failed");
+    return G4S_OK;
+}
+
 }  // namespace g4s
 
 #define HIP_OK G4S_HIP_OK
@@ -396,12 +416,7 @@ int g4s_create(const g4s_spec *spec, g4s_net **out)
     const size_t ns = n * (size_t)spec->nsyn;
     if ((rc = net->alloc(&syn, ns))) return bail(rc);
     d.syn = syn;
-    {
-        std::vector<uint2> tab(ns);
-        build_table(spec->seed, spec->n, d.n_exc, spec->nsyn, tab.data(), 0);
-        if (hipMemcpy(syn, tab.data(), ns * sizeof(uint2), hipMemcpyHostToDevice) != hipSuccess)
-            return bail(fail(G4S_ERR_HIP, "upload of the synapse table failed"));
-    }
+    if ((rc = build_synapses(spec, syn, net->stream))) return bail(rc);
     if ((rc = ensure_chunk(net, 1000))) return bail(rc);
     if ((rc = g4s_reset(net))) return bail(rc);
     *out = net;
@@ -706,6 +721,31 @@ int g4s_destroy(g4s_net *net)
     if (net->ev1) hipEventDestroy(net->ev1);
     if (net->stream) hipStreamDestroy(net->stream);
     delete net;
+    return G4S_OK;
+}
+
+int g4s_build_synapses_device(const g4s_spec *spec, uint32_t *post, float *w)
+{
+    int rc = validate_spec(spec);
+    if (rc) return rc;
+    if (!post || !w) return fail(G4S_ERR_ARG, "NULL output");
+    if (spec->nsyn > kDeviceBuildMaxNsyn) return fail(G4S_ERR_UNSUPPORTED, "device build needs Nsyn <= %d", kDeviceBuildMaxNsyn);
+    const size_t ns = (size_t)spec->n * (size_t)spec->nsyn;
+    uint2 *d = nullptr;
+    if (hipMalloc((void **)&d, ns * sizeof(uint2)) != hipSuccess) return fail(G4S_ERR_MEM, "hipMalloc(%zu)", ns * 8);
+    rc = build_synapses(spec, d, nullptr);
+    std::vector<uint2> tab;
+    if (rc == G4S_OK) {
+        tab.resize(ns);
+        if (hipMemcpy(tab.data(), d, ns * sizeof(uint2), hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(G4S_ERR_HIP, "copy of the device table failed");
+    }
+    hipFree(d);
+    if (rc) return rc;
+    for (size_t k = 0; k < ns; ++k) {
+        post[k] = tab[k].x;
+        w[k] = u2f(tab[k].y);
+    }
     return G4S_OK;
 }
 

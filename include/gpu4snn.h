@@ -151,6 +151,12 @@ int g4s_version(void);
  * g4s_create uploads).  `threads` <= 0 uses the hardware concurrency. */
 int g4s_build_synapses_host(const g4s_spec *spec, uint32_t *post, float *w, int32_t threads);
 
+/* The same table built by the device builder (used by g4s_create and the
+ * batched form whenever Nsyn <= 1024) and copied back; Nsyn > 1024 returns
+ * G4S_ERR_UNSUPPORTED.  No reference counterpart: the reference's device
+ * generator is never launched (cu:220-268, cu:915-953). */
+int g4s_build_synapses_device(const g4s_spec *spec, uint32_t *post, float *w);
+
 /* Device diagnostics: thalamic draws of neurons [0, n) at step t under
  * `seed` computed by a gfx950 kernel, and the Box-Muller transform of given
  * word pairs on the device. */
@@ -225,6 +231,68 @@ int g4s_batch_remove(g4s_batch *b, int32_t slot);
  * launches of the batched step kernel, grid = its largest grid). */
 int g4s_batch_timing(const g4s_batch *b, g4s_timing *out);
 int g4s_batch_destroy(g4s_batch *b);
+
+/* ------------------------------------------------------------------------
+ * Serving (no reference counterpart).  A server owns one batch and a worker
+ * thread that admits submitted requests in arrival order while their
+ * footprint (g4s_batch_footprint) fits the budget, runs one chunk of every
+ * admitted request per iteration (requests arriving meanwhile join at the
+ * next chunk: continuous batching), and retires finished requests, copying
+ * their counts (and raster) to host memory and freeing their device memory.
+ * All functions are thread-safe.  Times are seconds since server creation.
+ * ---------------------------------------------------------------------- */
+typedef struct g4s_server g4s_server;
+
+typedef struct g4s_server_opts {
+    int32_t accum;       /* G4S_ACCUM_* of every request                        */
+    int32_t chunk;       /* steps per chunk, 0 = 100                            */
+    int32_t slice_len;   /* as g4s_batch_opts.slice_len                         */
+    int32_t reserved;    /* must be 0                                           */
+    int64_t budget;      /* device bytes for resident requests, 0 = no limit   */
+} g4s_server_opts;
+
+#define G4S_REQ_PENDING 0
+#define G4S_REQ_ACTIVE  1
+#define G4S_REQ_DONE    2
+#define G4S_REQ_FAILED  3
+
+typedef struct g4s_request_info {
+    int32_t state;       /* G4S_REQ_*                                           */
+    int32_t error;       /* G4S_ERR_* of a failed request                       */
+    int64_t n;           /* neurons                                             */
+    int64_t steps;       /* steps requested                                     */
+    int64_t t;           /* steps simulated (set when done)                     */
+    int64_t spikes;      /* spikes (set when done)                              */
+    double  submit_s, admit_s, done_s;
+} g4s_request_info;
+
+typedef struct g4s_server_stats {
+    int64_t submitted, completed, failed, pending, active;
+    int64_t chunks;          /* chunks run                                      */
+    int64_t steps;           /* instance-steps simulated                        */
+    int64_t sim_steps_done;  /* steps of completed requests                     */
+    int64_t bytes_in_use, max_bytes, max_active;
+    double  busy_ms;         /* host time inside chunk calls                    */
+    double  device_ms;       /* device time of the chunks                       */
+    double  now_s;
+} g4s_server_stats;
+
+int g4s_server_create(const g4s_server_opts *opts, g4s_server **out);
+/* Queue a request; G4S_ERR_BUDGET if it can never fit the budget. */
+int g4s_server_submit(g4s_server *s, const g4s_spec *spec, int64_t steps, int keep_raster, int64_t *id);
+int g4s_server_info(g4s_server *s, int64_t id, g4s_request_info *info);
+/* Block until the request is done (G4S_OK) or failed (its error code);
+ * G4S_ERR_TIMEOUT after timeout_s (<= 0: no limit). */
+int g4s_server_wait(g4s_server *s, int64_t id, double timeout_s);
+/* Block until nothing is pending or active. */
+int g4s_server_drain(g4s_server *s, double timeout_s);
+int g4s_server_counts(g4s_server *s, int64_t id, int32_t *counts);
+int g4s_server_raster(g4s_server *s, int64_t id, uint32_t *words);
+/* Forget a finished request's results. */
+int g4s_server_release(g4s_server *s, int64_t id);
+int g4s_server_get_stats(g4s_server *s, g4s_server_stats *out);
+/* Stop the worker (requests still queued are dropped) and free everything. */
+int g4s_server_destroy(g4s_server *s);
 
 #ifdef __cplusplus
 }

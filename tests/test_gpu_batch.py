@@ -134,16 +134,21 @@ def test_batch_equals_single_network(g4s):
 
 @pytest.mark.parametrize("schedule", ["queue", "step"])
 def test_batch_atomic_first_steps_and_rates(g4s, oracle, schedule):
-    specs = [(2500, 1000, 51, 1.0, 1000), (2500, 1000, 52, 1.0, 1000), (4000, 500, 53, 1.2, 1000)]
-    with g4s.Batch("atomic", chunk=100, schedule=schedule) as b:
+    """First 100 steps identical per instance; the spike total of the three
+    instances over 4000 steps (~230k spikes, so decorrelated runs differ by
+    ~0.3% rms) within 1% of the oracle's."""
+    specs = [(2500, 1000, 51, 1.0, 4000), (2500, 1000, 52, 1.0, 4000), (4000, 500, 53, 1.2, 4000)]
+    with g4s.Batch("atomic", chunk=500, schedule=schedule) as b:
         slots = [b.add(n, ns, sd, st, thal_scale=sc) for (n, ns, sd, sc, st) in specs]
         b.run()
+        tot_gpu = tot_ref = 0
         for slot, (n, ns, sd, sc, st) in zip(slots, specs):
             _, oraster, ocounts = oracle_run(oracle, n, ns, sd, sc, st)
-            dense = g4s.raster_to_dense(b.raster(slot), n)
-            assert np.array_equal(dense[:100], oraster[:100]), (n, sd)
-            c = b.counts(slot)
-            assert abs(int(c.sum()) - int(ocounts.sum())) <= 0.01 * ocounts.sum(), (c.sum(), ocounts.sum())
+            dense = g4s.raster_to_dense(b.raster(slot)[:100], n)
+            assert np.array_equal(dense, oraster[:100]), (n, sd)
+            tot_gpu += int(b.counts(slot).sum())
+            tot_ref += int(ocounts.sum())
+        assert abs(tot_gpu - tot_ref) <= 0.01 * tot_ref, (tot_gpu, tot_ref)
 
 
 def test_batch_budget_admission(g4s):

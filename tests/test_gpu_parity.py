@@ -26,6 +26,21 @@ def cfg0(oracle):
     return _oracle_run(oracle, 2500, 1000, 1000)
 
 
+# Firing rates of atomic mode are compared over RATE_STEPS steps: once float
+# atomics reorder an addition the trajectory decorrelates from the oracle's,
+# and two independent 1000-step runs at N=2500 (~18k spikes) differ by about
+# 1% (1 sigma) on their own.  At 10000 steps (~180k spikes) the spread of
+# the difference is ~0.35%, so the 1% bar is about 3 sigma.
+RATE_STEPS = 10_000
+
+
+@pytest.fixture(scope="module")
+def cfg0_rate(oracle):
+    net = oracle.Network(2500, 1000, SEED)
+    _, counts, _ = net.run(RATE_STEPS, raster=False)
+    return int(counts.sum())
+
+
 def test_device_normals_bitwise(g4s, oracle):
     rng = np.random.default_rng(7)
     n = 1 << 24
@@ -74,12 +89,12 @@ def test_det_bitwise_cfg0(g4s, cfg0, alg):
 
 
 @pytest.mark.parametrize("alg", ALGS)
-def test_atomic_cfg0(g4s, cfg0, alg):
+def test_atomic_cfg0(g4s, cfg0, cfg0_rate, alg):
     onet, oraster, ocounts, _ = cfg0
-    steps = oraster.shape[0]
     with g4s.Network(2500, 1000, SEED) as net:
-        words, counts = net.run(steps, alg=alg, accum="atomic")
+        words, counts = net.run(100, alg=alg, accum="atomic")
         dense = g4s.raster_to_dense(words, 2500)
-        assert np.array_equal(dense[:100], oraster[:100]), "raster differs within the first 100 steps"
-        rate_gpu, rate_ref = counts.sum(), ocounts.sum()
+        assert np.array_equal(dense, oraster[:100]), "raster differs within the first 100 steps"
+        _, more = net.run(RATE_STEPS - 100, alg=alg, accum="atomic", raster=False)
+        rate_gpu, rate_ref = int(counts.sum()) + int(more.sum()), cfg0_rate
         assert abs(rate_gpu - rate_ref) <= 0.01 * rate_ref, (rate_gpu, rate_ref)
