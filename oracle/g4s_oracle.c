@@ -397,3 +397,79 @@ int orc_simulate_f64_forced(const orc_net *net, int32_t t0, int32_t steps,
     free(isyn);
     return 0;
 }
+
+/* One fp64 Izhikevich step of listed neuron i (cu:352-407 in double). */
+static int orc_f64_step(const orc_net *net, uint32_t t, int64_t i, double in, double *v, double *u)
+{
+    double z = (double)orc_thalamic(net->seed, t, (uint32_t)i);
+    double g = (i < net->n_exc) ? (double)net->ge : (double)net->gi;
+    double I = g * z + in;
+    double vv = *v, uu = *u;
+    for (int h = 0; h < 2; ++h)
+        vv += 0.5 * (0.04 * vv * vv + 5.0 * vv + 140.0 - uu + I);
+    uu += (double)net->a[i] * ((double)net->b[i] * vv - uu);
+    int sp = 0;
+    if (vv >= 30.0 || !isfinite(vv)) { vv = (double)net->c[i]; uu += (double)net->d[i]; sp = 1; }
+    *v = vv;
+    *u = uu;
+    return sp;
+}
+
+/* first index of the ascending row with target >= key */
+static int32_t orc_lower_bound(const uint32_t *row, int32_t len, uint32_t key)
+{
+    int32_t a = 0, b = len;
+    while (a < b) {
+        const int32_t m = (a + b) / 2;
+        if (row[m] < key) a = m + 1;
+        else b = m;
+    }
+    return a;
+}
+
+int orc_simulate_f64_forced_ranges(const orc_net *net, int32_t t0, int32_t steps,
+                                   const uint32_t *forced_words, int64_t words_per_row,
+                                   const int64_t *lo, const int64_t *hi, int32_t nranges,
+                                   double *v, double *u, double *v_trace, uint8_t *own_raster)
+{
+    const int64_t n = net->n;
+    int64_t nsub = 0;
+    int64_t base[64];
+    if (nranges < 1 || nranges > 64) return -1;
+    for (int32_t r = 0; r < nranges; ++r) {
+        if (lo[r] < 0 || hi[r] > n || lo[r] >= hi[r] || (r && lo[r] < hi[r - 1])) return -1;
+        base[r] = nsub;
+        nsub += hi[r] - lo[r];
+    }
+    double *isyn = (double *)calloc((size_t)nsub, sizeof(double));
+    if (!isyn) return -2;
+    for (int32_t s = 0; s < steps; ++s) {
+        const uint32_t t = (uint32_t)(t0 + s);
+        for (int32_t r = 0; r < nranges; ++r)
+            for (int64_t i = lo[r]; i < hi[r]; ++i) {
+                const int64_t k = base[r] + (i - lo[r]);
+                const int sp = orc_f64_step(net, t, i, isyn[k], &v[k], &u[k]);
+                isyn[k] = 0.0;
+                if (v_trace) v_trace[(size_t)s * (size_t)nsub + (size_t)k] = v[k];
+                if (own_raster) own_raster[(size_t)s * (size_t)nsub + (size_t)k] = (uint8_t)sp;
+            }
+        const uint32_t *fr = forced_words + (size_t)s * (size_t)words_per_row;
+        for (int64_t w = 0; w < words_per_row; ++w) {
+            uint32_t bits = fr[w];
+            while (bits) {
+                const int64_t i = w * 32 + __builtin_ctz(bits);
+                bits &= bits - 1u;
+                if (i >= n) break;
+                const uint32_t *row = net->post + (size_t)i * (size_t)net->nsyn;
+                const float *wr = net->w + (size_t)i * (size_t)net->nsyn;
+                for (int32_t r = 0; r < nranges; ++r) {
+                    int32_t e = orc_lower_bound(row, net->nsyn, (uint32_t)lo[r]);
+                    for (; e < net->nsyn && (int64_t)row[e] < hi[r]; ++e)
+                        isyn[base[r] + ((int64_t)row[e] - lo[r])] += (double)wr[e];
+                }
+            }
+        }
+    }
+    free(isyn);
+    return 0;
+}

@@ -111,6 +111,19 @@ int orc_simulate_f64_forced(const orc_net *net, int32_t t0, int32_t steps,
                             const int64_t *sub, int64_t nsub,
                             double *v, double *u, double *v_trace, uint8_t *own_raster);
 
+/* Same integration for a subsample given as ascending disjoint id ranges
+ * [lo[r], hi[r]), with the forcing raster bit-packed (steps rows of
+ * words_per_row words; bit i & 31 of word i >> 5 = neuron i spiked).  Each
+ * spiking row contributes through a binary search of its sorted targets per
+ * range, so N = 1e6 networks are cheap.  Inputs are added in ascending
+ * presynaptic order, as in orc_simulate_f64_forced.
+ * Pinned: bit-identical to orc_simulate_f64_forced on the same subsample
+ * (tests/test_oracle_model.py). */
+int orc_simulate_f64_forced_ranges(const orc_net *net, int32_t t0, int32_t steps,
+                                   const uint32_t *forced_words, int64_t words_per_row,
+                                   const int64_t *lo, const int64_t *hi, int32_t nranges,
+                                   double *v, double *u, double *v_trace, uint8_t *own_raster);
+
 #ifdef __cplusplus
 }
 #endif

@@ -69,6 +69,9 @@ def lib():
         L.orc_simulate_f64_forced.argtypes = [C.POINTER(OrcNet), C.c_int32, C.c_int32, _u8p, _i64p,
                                               C.c_int64, _f64p, _f64p, C.c_void_p, C.c_void_p]
         L.orc_simulate_f64_forced.restype = C.c_int
+        L.orc_simulate_f64_forced_ranges.argtypes = [C.POINTER(OrcNet), C.c_int32, C.c_int32, C.c_void_p, C.c_int64,
+                                                     _i64p, _i64p, C.c_int32, _f64p, _f64p, C.c_void_p, C.c_void_p]
+        L.orc_simulate_f64_forced_ranges.restype = C.c_int
         _lib = L
     return _lib
 
@@ -134,18 +137,25 @@ def n_exc_of(n):
 class Network:
     """One network as the oracle sees it: parameters, state, synapse table."""
 
-    def __init__(self, n, nsyn, seed, regime=1, thal_scale=1.0, n_exc=None):
+    def __init__(self, n, nsyn, seed, regime=1, thal_scale=1.0, n_exc=None, table=None):
+        """table: optional (post, w) arrays to use instead of building them
+        (e.g. a table already checked equal to orc_build_synapses')."""
         self.n, self.nsyn, self.seed = int(n), int(nsyn), int(seed)
         self.n_exc = n_exc_of(self.n) if n_exc is None else int(n_exc)
         self.ge, self.gi = gains(regime, thal_scale)
         L = lib()
         self.a, self.b, self.c, self.d, self.v, self.u = (np.empty(self.n, np.float32) for _ in range(6))
         L.orc_init_params(self.seed, self.n, self.n_exc, self.a, self.b, self.c, self.d, self.v, self.u)
-        self.post = np.empty(self.n * self.nsyn, np.uint32)
-        self.w = np.empty(self.n * self.nsyn, np.float32)
-        rc = L.orc_build_synapses(self.seed, self.n, self.n_exc, self.nsyn, self.post, self.w)
-        if rc != 0:
-            raise ValueError(f"orc_build_synapses failed ({rc}) for N={n} Nsyn={nsyn}")
+        if table is not None:
+            self.post = np.ascontiguousarray(table[0], np.uint32)
+            self.w = np.ascontiguousarray(table[1], np.float32)
+            assert self.post.size == self.n * self.nsyn and self.w.size == self.post.size
+        else:
+            self.post = np.empty(self.n * self.nsyn, np.uint32)
+            self.w = np.empty(self.n * self.nsyn, np.float32)
+            rc = L.orc_build_synapses(self.seed, self.n, self.n_exc, self.nsyn, self.post, self.w)
+            if rc != 0:
+                raise ValueError(f"orc_build_synapses failed ({rc}) for N={n} Nsyn={nsyn}")
         self.isyn = np.zeros(self.n, np.float32)
         self.nospks = np.zeros(self.n, np.int32)
         self.t = 0
@@ -167,6 +177,25 @@ class Network:
             raise RuntimeError(f"orc_simulate failed ({rc})")
         self.t += steps
         return r, counts, last
+
+    def run_f64_forced_ranges(self, forced_words, ranges, t0, v0, u0):
+        """fp64 teacher-forced integration of the neurons in `ranges` (list of
+        ascending [lo, hi)) forced by a bit-packed raster [steps, words]."""
+        fw = np.ascontiguousarray(forced_words, np.uint32)
+        steps = fw.shape[0]
+        lo = np.array([r[0] for r in ranges], np.int64)
+        hi = np.array([r[1] for r in ranges], np.int64)
+        nsub = int((hi - lo).sum())
+        v = np.ascontiguousarray(v0, np.float64).copy()
+        u = np.ascontiguousarray(u0, np.float64).copy()
+        assert v.size == nsub and u.size == nsub
+        tr = np.zeros((steps, nsub), np.float64)
+        own = np.zeros((steps, nsub), np.uint8)
+        rc = lib().orc_simulate_f64_forced_ranges(C.byref(self._cnet()), t0, steps, fw.ctypes.data, fw.shape[1],
+                                                  lo, hi, len(ranges), v, u, tr.ctypes.data, own.ctypes.data)
+        if rc != 0:
+            raise RuntimeError(f"orc_simulate_f64_forced_ranges failed ({rc})")
+        return tr, own
 
     def run_f64_forced(self, forced_raster, sub, t0, v0, u0):
         """fp64 teacher-forced integration of neurons `sub` from state (v0, u0) at step t0."""

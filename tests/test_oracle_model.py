@@ -206,3 +206,20 @@ def test_f64_forced_matches_f32_short_horizon(oracle):
     assert np.array_equal(own, r[:, sub])
     # measured 4e-3 mV after 30 steps (fp32 rounding grows on the spike upswing)
     assert np.abs(tr[-1] - net.v[sub]).max() < 2e-2
+
+
+def test_f64_forced_ranges_equals_list_form(oracle):
+    """The range/bit-packed form reproduces orc_simulate_f64_forced exactly
+    (same inputs in the same order) on two ranges spanning E and I."""
+    n, nsyn, steps = 700, 150, 80
+    net = oracle.Network(n, nsyn, seed=9, regime=2)
+    v0, u0 = net.v.copy(), net.u.copy()
+    r, _, _ = net.run(steps)
+    ranges = [(10, 90), (530, 700)]
+    sub = np.concatenate([np.arange(a, b) for a, b in ranges])
+    words = np.packbits(np.pad(r, ((0, 0), (0, (-n) % 32))), axis=1, bitorder="little").view(np.uint32)
+    f = oracle.Network(n, nsyn, seed=9, regime=2, table=(net.post, net.w))
+    tr1, own1 = f.run_f64_forced(r, sub, 0, v0[sub], u0[sub])
+    tr2, own2 = f.run_f64_forced_ranges(words, ranges, 0, v0[sub], u0[sub])
+    assert np.array_equal(tr1, tr2) and np.array_equal(own1, own2)
+    assert np.array_equal(own1, r[:, sub])
