@@ -86,6 +86,7 @@ class ServerStats(C.Structure):
 
 
 _vp = C.c_void_p
+CHUNK_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int64, C.c_int32, C.POINTER(C.c_uint32), C.POINTER(C.c_int32))
 
 
 def lib():
@@ -101,6 +102,7 @@ def lib():
             "g4s_set_regime": ([_vp, C.c_int, C.c_float], C.c_int),
             "g4s_run": ([_vp, C.c_int, C.c_int, C.c_int32, _vp, _vp, C.POINTER(RunOpts)], C.c_int),
             "g4s_get_timing": ([_vp, C.POINTER(Timing)], C.c_int),
+            "g4s_run_stream": ([_vp, C.c_int, C.c_int, C.c_int32, CHUNK_FN, _vp, C.POINTER(RunOpts)], C.c_int),
             "g4s_current_step": ([_vp], C.c_int64),
             "g4s_get_state": ([_vp, _vp, _vp, _vp, _vp], C.c_int),
             "g4s_get_params": ([_vp, _vp, _vp, _vp, _vp], C.c_int),
@@ -250,6 +252,22 @@ class Network:
         opts = RunOpts(block, slices, chunk, graph, int(bool(kernel_events)))
         _check(lib().g4s_run(self._h, a, m, int(steps), _ptr(r), _ptr(counts), C.byref(opts)))
         return r, counts
+
+    def run_stream(self, steps, on_chunk, alg="AB", accum="atomic", chunk=0, block=0, slices=0):
+        """g4s_run_stream: on_chunk(t0, raster_words[rows, words], counts[rows])
+        is called per chunk (copies of the library's buffers) while the next
+        chunk runs; returning True stops the run."""
+        w = words_of(self.n)
+
+        def cb(_user, t0, rows, raster, counts):
+            r = np.ctypeslib.as_array(raster, shape=(rows * w,)).reshape(rows, w).copy()
+            c = np.ctypeslib.as_array(counts, shape=(rows,)).copy()
+            return 1 if on_chunk(int(t0), r, c) else 0
+
+        fn = CHUNK_FN(cb)
+        a = ALG[alg] if isinstance(alg, str) else int(alg)
+        opts = RunOpts(block, slices, chunk, 0, 0)
+        _check(lib().g4s_run_stream(self._h, a, ACCUM[accum], int(steps), fn, None, C.byref(opts)))
 
     def autotune(self, accum="atomic", trial_steps=100, force=False):
         """Fastest scheme + tiling for this network (g4s_autotune); resets the network."""

@@ -12,8 +12,10 @@
 //   neuroninfoafter.csv  the final state (cu:1318-1327; the reference wrote
 //                        the pre-simulation copy there)
 // Optional flags select a subset or change the accumulation mode/seed.
-// Per-step times in computetime.csv are the device time of the step's chunk
-// divided by its steps (steps run in device-resident chunks).
+// Spike rows are streamed to spiketimes.csv chunk by chunk while the device
+// runs the next chunk (g4s_run_stream).  Per-step times in computetime.csv
+// are the run's device time divided by its steps (steps run in
+// device-resident chunks).
 #include "gpu4snn.h"
 
 #include <sys/stat.h>
@@ -62,6 +64,39 @@ void write_neuroninfo(const std::string &path, int64_t n, const std::vector<floa
         f << a[i] << " " << b[i] << " " << c[i] << " " << d[i] << " " << v[i] << " " << u[i] << " " << I[i] << " "
           << i << " " << nos[i] << "\n";
     *ok = true;
+}
+
+// Streaming sink: spike rows of each chunk go to spiketimes.csv while the
+// device runs the next chunk; counts are kept for computetime.csv.
+struct Sink {
+    std::ofstream *spikes;
+    std::vector<int32_t> counts;
+    int64_t words;
+    long long total;
+};
+
+int write_chunk(void *user, int64_t t0, int32_t rows, const uint32_t *raster, const int32_t *counts)
+{
+    Sink *k = (Sink *)user;
+    std::string buf;
+    buf.reserve(4096);
+    char line[48];
+    for (int32_t r = 0; r < rows; ++r) {
+        const uint32_t *row = raster + (size_t)r * (size_t)k->words;
+        for (int64_t w = 0; w < k->words; ++w) {
+            uint32_t bits = row[w];
+            while (bits) {
+                const int b = __builtin_ctz(bits);
+                const int len = snprintf(line, sizeof(line), "%lld %lld\n", (long long)(t0 + r), (long long)(w * 32 + b));
+                buf.append(line, (size_t)len);
+                bits &= bits - 1;
+            }
+        }
+        k->counts.push_back(counts[r]);
+        k->total += counts[r];
+    }
+    k->spikes->write(buf.data(), (std::streamsize)buf.size());
+    return k->spikes->good() ? 0 : 1;
 }
 
 int check(int rc, const char *what)
@@ -138,9 +173,8 @@ int main(int argc, char **argv)
     check(g4s_create(&spec, &net), "g4s_create");
 
     const int64_t words = (N + 31) / 32;
-    const int chunk = 1000;
-    std::vector<uint32_t> raster((size_t)chunk * (size_t)words);
-    std::vector<int32_t> counts(chunk);
+    g4s_run_opts ropts{};
+    ropts.chunk = 1000;
     std::vector<float> a(N), b(N), c(N), d(N), v(N), u(N), I(N);
     std::vector<int32_t> nos(N);
     mkdir(outdir.c_str(), S_IRWXU | S_IRWXG | S_IROTH | S_IXOTH);
@@ -170,30 +204,17 @@ int main(int argc, char **argv)
             ctime << "TimeStep SpksPerStep TimeKernelUpdate TimeSpent" << "\n";
 
             std::cout << "\nStarting simulation and timer...\n";
-            double dev_ms = 0.0, wall_ms = 0.0;
-            long long total = 0;
-            for (long s0 = 0; s0 < iters; s0 += chunk) {
-                const int cs = (int)std::min<long>(chunk, iters - s0);
-                check(g4s_run(net, alg, mode, cs, raster.data(), counts.data(), nullptr), "g4s_run");
-                g4s_timing tm{};
-                g4s_get_timing(net, &tm);
-                dev_ms += tm.device_ms;
-                wall_ms += tm.wall_ms;
-                const double per = cs ? tm.device_ms / cs : 0.0;
-                for (int r = 0; r < cs; ++r) {
-                    const uint32_t *row = raster.data() + (size_t)r * (size_t)words;
-                    for (int64_t w = 0; w < words; ++w) {
-                        uint32_t bits = row[w];
-                        while (bits) {
-                            const int k = __builtin_ctz(bits);
-                            spikes << (s0 + r) << " " << (w * 32 + k) << "\n";
-                            bits &= bits - 1;
-                        }
-                    }
-                    ctime << (s0 + r) << " " << counts[r] << " " << per << " " << per << "\n";
-                    total += counts[r];
-                }
-            }
+            Sink sink{&spikes, {}, words, 0};
+            sink.counts.reserve((size_t)iters);
+            if (iters > 0)
+                check(g4s_run_stream(net, alg, mode, (int32_t)iters, write_chunk, &sink, &ropts), "g4s_run_stream");
+            g4s_timing tm{};
+            g4s_get_timing(net, &tm);
+            const double dev_ms = iters > 0 ? tm.device_ms : 0.0, wall_ms = iters > 0 ? tm.wall_ms : 0.0;
+            const long long total = sink.total;
+            const double per = iters > 0 ? dev_ms / (double)iters : 0.0;
+            for (size_t r = 0; r < sink.counts.size(); ++r)
+                ctime << r << " " << sink.counts[r] << " " << per << " " << per << "\n";
             std::cout << "\nEnd of simulation...\n";
             std::cout << "Simulation took: " << wall_ms << " ms.\n";
             std::cout << "Without data transfers: " << dev_ms << " ms.\n";

@@ -87,11 +87,16 @@ struct g4s_net {
     g4s_spec spec{};
     DevNet d{};
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // Per staging buffer k (chunks alternate between two so that the host
+    // can consume chunk c while chunk c+1 runs): events around the chunk's
+    // step launches and at the end of its copies, pinned copies of its raster
+    // rows, control words, error flag and step info.
+    hipEvent_t ev0[2] = {nullptr, nullptr}, ev1[2] = {nullptr, nullptr}, done_ev[2] = {nullptr, nullptr};
     int64_t t = 0;
-    int chunk_cap = 0;          // raster rows / ctl entries allocated
-    uint32_t *h_raster = nullptr;  // pinned staging
-    int32_t *h_ctl = nullptr;      // pinned staging
+    int chunk_cap = 0;             // raster rows / ctl entries allocated
+    uint32_t *h_raster[2] = {nullptr, nullptr};
+    int32_t *h_ctl[2] = {nullptr, nullptr};
+    int32_t *h_err = nullptr;      // [2]
     bool det_ready = false;
     bool slices_ready = false;
     // Per-step AB launches captured once per chunk length and device layout.
@@ -99,7 +104,7 @@ struct g4s_net {
     hipGraphExec_t graph_exec = nullptr;
     int graph_steps = 0, graph_block = 0;
     int64_t gen = 0, graph_gen = -1;  // gen: bumped when DevNet changes
-    int32_t *h_tinfo = nullptr;       // pinned staging of d.tinfo
+    int32_t *h_tinfo = nullptr;       // pinned staging of d.tinfo, [2][2]
     // isyn0/isyn1 hold the pending input only after the global-atomic schemes
     // (N, S, AP atomic); the slice and gather schemes derive the input from
     // the previous step's queue or bitmap instead.
@@ -182,9 +187,13 @@ int ensure_chunk(g4s_net *net, int chunk)
     if (chunk <= net->chunk_cap) return G4S_OK;
     if (net->d.raster) { hipFree(net->d.raster); net->allocs.erase(std::find(net->allocs.begin(), net->allocs.end(), (void *)net->d.raster)); }
     if (net->d.ctl) { hipFree(net->d.ctl); net->allocs.erase(std::find(net->allocs.begin(), net->allocs.end(), (void *)net->d.ctl)); }
-    if (net->h_raster) hipHostFree(net->h_raster);
-    if (net->h_ctl) hipHostFree(net->h_ctl);
-    net->d.raster = nullptr; net->d.ctl = nullptr; net->h_raster = nullptr; net->h_ctl = nullptr;
+    for (int k = 0; k < 2; ++k) {
+        if (net->h_raster[k]) hipHostFree(net->h_raster[k]);
+        if (net->h_ctl[k]) hipHostFree(net->h_ctl[k]);
+        net->h_raster[k] = nullptr;
+        net->h_ctl[k] = nullptr;
+    }
+    net->d.raster = nullptr; net->d.ctl = nullptr;
     int rc;
     if ((rc = net->alloc(&net->d.raster, (size_t)chunk * net->d.words))) return rc;
     if ((rc = net->alloc(&net->d.ctl, (size_t)chunk * kCtlStride))) return rc;
@@ -194,8 +203,10 @@ int ensure_chunk(g4s_net *net, int chunk)
         net->d.bcnt = nullptr;
     }
     if (net->d.P > 0 && (rc = net->alloc(&net->d.bcnt, (size_t)chunk * net->d.P))) return rc;
-    HIP_OK(hipHostMalloc((void **)&net->h_raster, (size_t)chunk * net->d.words * sizeof(uint32_t)));
-    HIP_OK(hipHostMalloc((void **)&net->h_ctl, (size_t)chunk * kCtlStride * sizeof(int32_t)));
+    for (int k = 0; k < 2; ++k) {
+        HIP_OK(hipHostMalloc((void **)&net->h_raster[k], (size_t)chunk * net->d.words * sizeof(uint32_t)));
+        HIP_OK(hipHostMalloc((void **)&net->h_ctl[k], (size_t)chunk * kCtlStride * sizeof(int32_t)));
+    }
     net->chunk_cap = chunk;
     net->gen += 1;
     return G4S_OK;
@@ -394,8 +405,10 @@ int g4s_create(const g4s_spec *spec, g4s_net **out)
     auto bail = [&](int code) { g4s_destroy(net); return code; };
     if (hipStreamCreateWithFlags(&net->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(G4S_ERR_HIP, "hipStreamCreate failed"));
-    if (hipEventCreate(&net->ev0) != hipSuccess || hipEventCreate(&net->ev1) != hipSuccess)
-        return bail(fail(G4S_ERR_HIP, "hipEventCreate failed"));
+    for (int k = 0; k < 2; ++k)
+        if (hipEventCreate(&net->ev0[k]) != hipSuccess || hipEventCreate(&net->ev1[k]) != hipSuccess ||
+            hipEventCreateWithFlags(&net->done_ev[k], hipEventDisableTiming) != hipSuccess)
+            return bail(fail(G4S_ERR_HIP, "hipEventCreate failed"));
     const size_t n = (size_t)spec->n;
     float **fields[] = {&d.v, &d.u, &d.a, &d.b, &d.c, &d.d, &d.lastI, &d.isyn0, &d.isyn1};
     for (float **f : fields)
@@ -410,7 +423,8 @@ int g4s_create(const g4s_spec *spec, g4s_net **out)
     int32_t *tinfo = nullptr;
     if ((rc = net->alloc(&tinfo, 2))) return bail(rc);
     d.tinfo = tinfo;
-    if (hipHostMalloc((void **)&net->h_tinfo, 2 * sizeof(int32_t)) != hipSuccess)
+    if (hipHostMalloc((void **)&net->h_tinfo, 4 * sizeof(int32_t)) != hipSuccess ||
+        hipHostMalloc((void **)&net->h_err, 2 * sizeof(int32_t)) != hipSuccess)
         return bail(fail(G4S_ERR_MEM, "hipHostMalloc failed"));
     uint2 *syn = nullptr;
     const size_t ns = n * (size_t)spec->nsyn;
@@ -496,8 +510,10 @@ hipError_t launch_step(g4s_net *net, int alg, int accum, int block, int t, int r
 
 }  // namespace
 
-int g4s_run(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, int32_t *counts,
-            const g4s_run_opts *opts)
+namespace {
+
+int run_impl(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, int32_t *counts,
+             const g4s_run_opts *opts, g4s_chunk_fn fn, void *user)
 {
     if (!net) return fail(G4S_ERR_ARG, "net is NULL");
     if (alg < G4S_ALG_AP || alg > G4S_ALG_SKL) return fail(G4S_ERR_ARG, "unknown alg %d", alg);
@@ -565,52 +581,23 @@ int g4s_run(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, i
     int launches = 0, dom_grid = 0, kern_launches = 0;
     const int64_t t_end = net->t + steps;
     const int64_t words = d.words;
+    // A streaming run hands chunk c to the callback while chunk c+1 runs.
+    // AP sizes a chunk's first step from the previous chunk's last count,
+    // and event-bracketed runs collect per chunk: those run one at a time.
+    const bool overlap = fn && !net->kev_on && !(alg == G4S_ALG_AP && accum == G4S_ACCUM_ATOMIC);
+    struct Pending { int64_t c0; int cs; int t0; int buf; };
+    Pending pend[2];
+    int npend = 0;
 
-    for (int64_t c0 = 0; c0 < steps; c0 += chunk) {
-        const int cs = (int)std::min<int64_t>(chunk, steps - c0);
-        const int t0 = (int)(net->t);
-        HIP_OK(launch_fill_u32((uint32_t *)d.ctl, (size_t)cs * kCtlStride, 0u, net->stream));
-        HIP_OK(launch_fill_u32(d.bar, 1, 0u, net->stream));
-        if (alg == G4S_ALG_AB && slice) {
-            net->h_tinfo[0] = t0;
-            net->h_tinfo[1] = (int32_t)(t_end - 1);
-            HIP_OK(hipMemcpyAsync((void *)d.tinfo, net->h_tinfo, 2 * sizeof(int32_t), hipMemcpyHostToDevice, net->stream));
-        }
-        HIP_OK(hipEventRecord(net->ev0, net->stream));
-        const bool one_launch = alg == G4S_ALG_SKL;  // ev0..ev1 brackets exactly it
-        if (alg == G4S_ALG_AB && slice) {
-            if ((rc = run_ab_chunk(net, cs, use_graph, sblock, det))) return rc;
-            launches += cs;
-            dom_grid = d.P;
-        } else if (alg == G4S_ALG_SKL) {
-            hipError_t e;
-            e = launch_skl_slice(d, t0, cs, (int)(t_end - 1), det, L);
-            if (e != hipSuccess) return fail(G4S_ERR_HIP, "SKL cooperative launch (grid %d): %s", L.grid, hipGetErrorString(e));
-            launches += 1;
-            dom_grid = L.grid;
-        } else {
-            for (int rel = 0; rel < cs; ++rel) {
-                const int t = t0 + rel;
-                hipError_t e = launch_step(net, alg, accum, block, t, rel, t == t_end - 1, &launches, &L, &rc);
-                if (rc) return rc;
-                if (e != hipSuccess) return fail(G4S_ERR_HIP, "step %d launch (alg %d, grid %d): %s", t, alg, L.grid, hipGetErrorString(e));
-                dom_grid = L.grid;
-            }
-        }
-        HIP_OK(hipEventRecord(net->ev1, net->stream));
-        if (slice) HIP_OK(launch_reduce_counts(d, cs, net->stream));
-        HIP_OK(hipMemcpyAsync(net->h_ctl, d.ctl, (size_t)cs * kCtlStride * sizeof(int32_t), hipMemcpyDeviceToHost, net->stream));
-        if (raster)
-            HIP_OK(hipMemcpyAsync(net->h_raster, d.raster, (size_t)cs * words * sizeof(uint32_t), hipMemcpyDeviceToHost, net->stream));
-        int32_t err = 0;
-        HIP_OK(hipMemcpyAsync(&err, d.err, sizeof(int32_t), hipMemcpyDeviceToHost, net->stream));
-        HIP_OK(hipStreamSynchronize(net->stream));
-        if (err) return fail(G4S_ERR_TIMEOUT, "a bounded device wait expired at chunk starting step %d", t0);
+    // consume the oldest enqueued chunk
+    auto finish = [&](const Pending &pc) -> int {
+        HIP_OK(hipEventSynchronize(net->done_ev[pc.buf]));
+        if (net->h_err[pc.buf]) return fail(G4S_ERR_TIMEOUT, "a bounded device wait expired at chunk starting step %d", pc.t0);
         float ms = 0.0f;
-        HIP_OK(hipEventElapsedTime(&ms, net->ev0, net->ev1));
+        HIP_OK(hipEventElapsedTime(&ms, net->ev0[pc.buf], net->ev1[pc.buf]));
         dev_ms += ms;
         if (net->kev_on) {
-            if (one_launch) {
+            if (alg == G4S_ALG_SKL) {  // ev0..ev1 brackets exactly the one launch
                 kern_ms += ms;
                 kern_launches += 1;
             } else {
@@ -618,17 +605,91 @@ int g4s_run(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, i
                 HIP_OK(kev_collect(net, &kern_ms));
             }
         }
+        const int32_t *hc = net->h_ctl[pc.buf];
         int64_t mx = 0;
-        for (int r = 0; r < cs; ++r) {
-            const int32_t cnt = net->h_ctl[r * kCtlStride + kCtlCount];
+        for (int r = 0; r < pc.cs; ++r) {
+            const int32_t cnt = hc[r * kCtlStride + kCtlCount];
             spikes += cnt;
             mx = std::max<int64_t>(mx, cnt);
-            if (counts) counts[c0 + r] = cnt;
+            if (counts) counts[pc.c0 + r] = cnt;
         }
-        net->last_count = net->h_ctl[(cs - 1) * kCtlStride + kCtlCount];
+        net->last_count = hc[(pc.cs - 1) * kCtlStride + kCtlCount];
         net->est_count = std::max<int64_t>(64, mx + mx / 4);
-        if (raster) memcpy(raster + (size_t)c0 * words, net->h_raster, (size_t)cs * words * sizeof(uint32_t));
+        if (raster) memcpy(raster + (size_t)pc.c0 * words, net->h_raster[pc.buf], (size_t)pc.cs * words * sizeof(uint32_t));
+        if (fn) {
+            std::vector<int32_t> cc((size_t)pc.cs);
+            for (int r = 0; r < pc.cs; ++r) cc[(size_t)r] = hc[r * kCtlStride + kCtlCount];
+            const int urc = fn(user, pc.t0, pc.cs, net->h_raster[pc.buf], cc.data());
+            if (urc) return fail(G4S_ERR_STATE, "stream callback returned %d at step %d", urc, pc.t0);
+        }
+        return G4S_OK;
+    };
+    // drain what is still in flight when the run stops early
+    auto abandon = [&](int code) {
+        for (int k = 0; k < npend; ++k) hipEventSynchronize(net->done_ev[pend[k].buf]);
+        return code;
+    };
+
+    int chunk_no = 0;
+    for (int64_t c0 = 0; c0 < steps; c0 += chunk, ++chunk_no) {
+        const int cs = (int)std::min<int64_t>(chunk, steps - c0);
+        const int t0 = (int)(net->t);
+        const int buf = chunk_no & 1;
+        HIP_OK(launch_fill_u32((uint32_t *)d.ctl, (size_t)cs * kCtlStride, 0u, net->stream));
+        HIP_OK(launch_fill_u32(d.bar, 1, 0u, net->stream));
+        if (alg == G4S_ALG_AB && slice) {
+            int32_t *ti = net->h_tinfo + 2 * buf;
+            ti[0] = t0;
+            ti[1] = (int32_t)(t_end - 1);
+            HIP_OK(hipMemcpyAsync((void *)d.tinfo, ti, 2 * sizeof(int32_t), hipMemcpyHostToDevice, net->stream));
+        }
+        HIP_OK(hipEventRecord(net->ev0[buf], net->stream));
+        if (alg == G4S_ALG_AB && slice) {
+            if ((rc = run_ab_chunk(net, cs, use_graph, sblock, det))) return abandon(rc);
+            launches += cs;
+            dom_grid = d.P;
+        } else if (alg == G4S_ALG_SKL) {
+            hipError_t e;
+            e = launch_skl_slice(d, t0, cs, (int)(t_end - 1), det, L);
+            if (e == hipErrorIllegalState)
+                return abandon(fail(G4S_ERR_HIP, "SKL cooperative launch refused (%s); observed when another process "
+                                    "on this device has made cooperative launches", hipGetErrorString(e)));
+            if (e != hipSuccess) return abandon(fail(G4S_ERR_HIP, "SKL cooperative launch (grid %d): %s", L.grid, hipGetErrorString(e)));
+            launches += 1;
+            dom_grid = L.grid;
+        } else {
+            for (int rel = 0; rel < cs; ++rel) {
+                const int t = t0 + rel;
+                hipError_t e = launch_step(net, alg, accum, block, t, rel, t == t_end - 1, &launches, &L, &rc);
+                if (rc) return abandon(rc);
+                if (e == hipErrorIllegalState && alg == G4S_ALG_AP)
+                    return abandon(fail(G4S_ERR_HIP, "step %d: cooperative launch refused (%s); observed when another "
+                                        "process on this device has made cooperative launches", t, hipGetErrorString(e)));
+                if (e != hipSuccess) return abandon(fail(G4S_ERR_HIP, "step %d launch (alg %d, grid %d): %s", t, alg, L.grid, hipGetErrorString(e)));
+                dom_grid = L.grid;
+            }
+        }
+        HIP_OK(hipEventRecord(net->ev1[buf], net->stream));
+        if (slice) HIP_OK(launch_reduce_counts(d, cs, net->stream));
+        HIP_OK(hipMemcpyAsync(net->h_ctl[buf], d.ctl, (size_t)cs * kCtlStride * sizeof(int32_t), hipMemcpyDeviceToHost, net->stream));
+        if (raster || fn)
+            HIP_OK(hipMemcpyAsync(net->h_raster[buf], d.raster, (size_t)cs * words * sizeof(uint32_t), hipMemcpyDeviceToHost, net->stream));
+        HIP_OK(hipMemcpyAsync(&net->h_err[buf], d.err, sizeof(int32_t), hipMemcpyDeviceToHost, net->stream));
+        HIP_OK(hipEventRecord(net->done_ev[buf], net->stream));
         net->t += cs;
+        pend[npend++] = Pending{c0, cs, t0, buf};
+        if (!overlap || npend == 2) {
+            rc = finish(pend[0]);
+            pend[0] = pend[1];
+            npend -= 1;
+            if (rc) return abandon(rc);
+        }
+    }
+    while (npend > 0) {
+        rc = finish(pend[0]);
+        pend[0] = pend[1];
+        npend -= 1;
+        if (rc) return abandon(rc);
     }
     net->timing.device_ms = dev_ms;
     net->timing.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
@@ -640,6 +701,21 @@ int g4s_run(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, i
     net->timing.kernel_launches = kern_launches;
     net->kev_on = false;
     return G4S_OK;
+}
+
+}  // namespace
+
+int g4s_run(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, int32_t *counts,
+            const g4s_run_opts *opts)
+{
+    return run_impl(net, alg, accum, steps, raster, counts, opts, nullptr, nullptr);
+}
+
+int g4s_run_stream(g4s_net *net, int alg, int accum, int32_t steps, g4s_chunk_fn fn, void *user,
+                   const g4s_run_opts *opts)
+{
+    if (!fn) return fail(G4S_ERR_ARG, "callback is NULL");
+    return run_impl(net, alg, accum, steps, nullptr, nullptr, opts, fn, user);
 }
 
 int g4s_set_regime(g4s_net *net, int regime, float thal_scale)
@@ -714,11 +790,17 @@ int g4s_destroy(g4s_net *net)
     drop_graph(net);
     for (void *p : net->allocs) hipFree(p);
     if (net->h_tinfo) hipHostFree(net->h_tinfo);
-    if (net->h_raster) hipHostFree(net->h_raster);
-    if (net->h_ctl) hipHostFree(net->h_ctl);
+    if (net->h_err) hipHostFree(net->h_err);
+    for (int k = 0; k < 2; ++k) {
+        if (net->h_raster[k]) hipHostFree(net->h_raster[k]);
+        if (net->h_ctl[k]) hipHostFree(net->h_ctl[k]);
+    }
     for (hipEvent_t e : net->kev) hipEventDestroy(e);
-    if (net->ev0) hipEventDestroy(net->ev0);
-    if (net->ev1) hipEventDestroy(net->ev1);
+    for (int k = 0; k < 2; ++k) {
+        if (net->ev0[k]) hipEventDestroy(net->ev0[k]);
+        if (net->ev1[k]) hipEventDestroy(net->ev1[k]);
+        if (net->done_ev[k]) hipEventDestroy(net->done_ev[k]);
+    }
     if (net->stream) hipStreamDestroy(net->stream);
     delete net;
     return G4S_OK;

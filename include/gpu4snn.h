@@ -101,6 +101,17 @@ int g4s_reset(g4s_net *net);
 int g4s_run(g4s_net *net, int alg, int accum, int32_t steps,
             uint32_t *raster, int32_t *counts, const g4s_run_opts *opts);
 
+/* Streaming form of g4s_run: after each chunk (opts->chunk steps) the
+ * callback receives its raster rows (rows x ceil(N/32) words, layout as
+ * above) and per-step counts, starting at step t0; both buffers are valid
+ * only during the call.  The next chunk already runs on the device while the
+ * callback works (except for AP in atomic mode and kernel_events runs), so
+ * writing the reference's spiketimes.csv rows (cu:1256-1261) overlaps the
+ * simulation.  A nonzero return stops the run (G4S_ERR_STATE). */
+typedef int (*g4s_chunk_fn)(void *user, int64_t t0, int32_t rows, const uint32_t *raster, const int32_t *counts);
+int g4s_run_stream(g4s_net *net, int alg, int accum, int32_t steps, g4s_chunk_fn fn, void *user,
+                   const g4s_run_opts *opts);
+
 int g4s_get_timing(const g4s_net *net, g4s_timing *out);
 
 /* Change the firing regime / thalamic scale of an existing network without
