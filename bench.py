@@ -37,8 +37,11 @@ def step_bytes(n, nsyn, spikes):
 def profiled_traffic(kernel, alg, accum, n, nsyn, regime, slices, block):
     """HBM bytes per launch of `kernel` from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary of the same workload and plan, or None."""
+    return traffic_of(f"{kernel}|{alg}|{accum}|n={n}|nsyn={nsyn}|regime={regime}|slices={slices}|block={block}")
+
+
+def traffic_of(key):
     path = os.path.join(ROOT, "profiles", "traffic.json")
-    key = f"{kernel}|{alg}|{accum}|n={n}|nsyn={nsyn}|regime={regime}|slices={slices}|block={block}"
     try:
         with open(path) as f:
             rec = json.load(f).get(key)
@@ -87,6 +90,15 @@ def batch_specs(count, seed):
 def bench_batch(g, args):
     specs = batch_specs(args.batch, args.seed)
     total = args.warmup + args.steps
+    tuned = {}
+    if args.tune and not args.slice_len:
+        t_tune = time.perf_counter()
+        for (n, ns, sd, sc) in specs:
+            key = (n.bit_length() - 1, ns)
+            if key not in tuned:
+                tuned[key] = g.batch_tune(n, ns, sd, accum=args.accum, thal_scale=1.15, trial_steps=60)["slice_len"]
+        print(f"tuned {len(tuned)} buckets in {time.perf_counter() - t_tune:.1f} s: "
+              f"{sorted(tuned.items())}", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     b = g.Batch(args.accum, chunk=args.chunk, slice_len=args.slice_len, schedule=args.schedule)
     slots = [b.add(n, ns, sd, total, thal_scale=sc, keep_raster=False) for (n, ns, sd, sc) in specs]
@@ -134,14 +146,19 @@ def bench_batch(g, args):
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded Philox networks)",
         "config": {"workload": f"{len(specs)} networks, N log-uniform [1e3, 5e4], Nsyn in {{100,500,1000}}, "
                                f"thalamic x U(0.8,1.5), {steps_timed} timed steps each",
-                   "accum": args.accum, "chunk": args.chunk, "slice_len": args.slice_len or "per instance",
+                   "accum": args.accum, "chunk": args.chunk,
+                   "slice_len": args.slice_len or ("tuned per bucket" if tuned else "min(512, N)"),
+                   "tuned": {f"log2N={k[0]},Nsyn={k[1]}": v for k, v in sorted(tuned.items())} or None,
                    "schedule": args.schedule, "neurons": sum(x[0] for x in specs),
                    "synapses": sum(x[0] * x[1] for x in specs), "seed": args.seed},
         "events_per_s": events / wall,
         "plain": {"sim_s_per_s": p_sim / p_wall, "sample": len(idx), "scheme": "AB"},
         "speedup_vs_plain": (sim_s / wall) / (p_sim / p_wall),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_batch_step",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic_of(f"k_batch_step|batch{len(specs)}|{args.accum}|chunk={args.chunk}|slice_len=512")
+                     if not args.slice_len and not args.tune and args.schedule == "step" and args.seed == 20221 else None,
+                     "kernel": "k_batch_step",
                      "bytes_per_launch": byts / max(1, launches),
                      "avg_launch_us": dev_ms * 1e3 / max(1, launches)},
     }
@@ -214,6 +231,37 @@ def bench_serve(g, args):
     }
 
 
+def extras(g, args):
+    """Batched and served summaries beside the single-network line: 256
+    networks in one batch (det mode, 1000 steps), then open-loop serving at
+    50 req/s for 20 s and an overload run at 400 req/s for 10 s."""
+    import copy
+    t0 = time.perf_counter()
+    a = copy.copy(args)
+    a.batch, a.accum, a.steps, a.warmup, a.plain_sample, a.slice_len, a.tune = 256, "det", 1000, 100, 8, 0, False
+    b = bench_batch(g, a)
+    print(f"batched done in {time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
+    s1 = copy.copy(a)
+    s1.serve, s1.rate = 20.0, 50.0
+    sv = bench_serve(g, s1)
+    s2 = copy.copy(a)
+    s2.serve, s2.rate = 10.0, 400.0
+    sat = bench_serve(g, s2)
+    print(f"served done in {time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
+    return {
+        "batched": {"workload": b["config"]["workload"], "accum": "det", "sim_s_per_s": b["value"],
+                    "events_per_s": b["events_per_s"], "ms_per_step": b["ms_per_step"],
+                    "plain_sim_s_per_s": b["plain"]["sim_s_per_s"], "speedup_vs_plain": b["speedup_vs_plain"],
+                    "roofline": b["roofline"]},
+        "served": {"workload": sv["config"]["workload"], "accum": "det", "p50_s": sv["latency_s"]["p50"],
+                   "p99_s": sv["latency_s"]["p99"], "sim_s_per_s": sv["value"],
+                   "offered_sim_s_per_s": sv["config"]["offered_sim_s_per_s"],
+                   "overload": {"rate_req_s": 400.0, "sim_s_per_s": sat["value"], "p50_s": sat["latency_s"]["p50"],
+                                "p99_s": sat["latency_s"]["p99"], "speedup_vs_plain": sat["speedup_vs_plain"]},
+                   "plain_sim_s_per_s": sat["plain"]["sim_s_per_s"]},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -245,6 +293,9 @@ def main():
     ap.add_argument("--slice-len", type=int, default=0, help="batched: neurons per slice, 0 = per instance")
     ap.add_argument("--schedule", default="step", choices=["step", "queue"])
     ap.add_argument("--plain-sample", type=int, default=32)
+    ap.add_argument("--tune", action="store_true", help="batched: tune the slice length per (N, Nsyn) bucket first")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batched (256 networks) and served (Poisson arrivals) summaries")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times with a plain copy kernel "
                          "(reference reading for the FETCH_SIZE/WRITE_SIZE counters) and exit")
@@ -354,10 +405,12 @@ def main():
                      "bytes_per_launch": per_launch_bytes, "avg_launch_us": avg_launch_s * 1e6,
                      "launches_timed": k_launches},
     }
+    net.close()
+    if not args.no_extras:
+        out.update(extras(g, args))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.n, args.nsyn, args.seed, args.regime, args.cpu_steps)
         out["cpu_baseline"]["cores"] = 1
-    net.close()
     print(json.dumps(out), flush=True)
 
 

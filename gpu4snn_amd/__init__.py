@@ -62,6 +62,11 @@ class BatchOpts(C.Structure):
                 ("budget", C.c_int64)]
 
 
+class BatchPlan(C.Structure):
+    _fields_ = [("slice_len", C.c_int32), ("copies", C.c_int32), ("us_per_inst_step", C.c_float),
+                ("rate_hz", C.c_float)]
+
+
 class InstInfo(C.Structure):
     _fields_ = [("n", C.c_int64), ("nsyn", C.c_int32), ("slice_len", C.c_int32), ("t", C.c_int64),
                 ("steps", C.c_int64), ("spikes", C.c_int64), ("bytes", C.c_int64), ("done", C.c_int32),
@@ -128,6 +133,7 @@ def lib():
             "g4s_batch_remove": ([_vp, C.c_int32], C.c_int),
             "g4s_batch_timing": ([_vp, C.POINTER(Timing)], C.c_int),
             "g4s_batch_destroy": ([_vp], C.c_int),
+            "g4s_batch_tune": ([C.c_int, C.POINTER(Spec), C.c_int32, C.c_int, C.POINTER(BatchPlan)], C.c_int),
             "g4s_server_create": ([C.POINTER(ServerOpts), C.POINTER(_vp)], C.c_int),
             "g4s_server_submit": ([_vp, C.POINTER(Spec), C.c_int64, C.c_int, C.POINTER(C.c_int64)], C.c_int),
             "g4s_server_info": ([_vp, C.c_int64, C.POINTER(RequestInfo)], C.c_int),
@@ -314,6 +320,14 @@ def footprint(n, nsyn, chunk=0, slice_len=0, seed=0, regime=1, thal_scale=1.0):
     if v < 0:
         raise G4SError(-1, lib().g4s_last_error().decode())
     return v
+
+
+def batch_tune(n, nsyn, seed=1, accum="det", thal_scale=1.0, trial_steps=100, force=False):
+    """g4s_batch_tune: slice length for the bucket of (N, Nsyn); returns a dict."""
+    p = BatchPlan()
+    spec = make_spec(n, nsyn, seed, 1, thal_scale)
+    _check(lib().g4s_batch_tune(ACCUM[accum], C.byref(spec), int(trial_steps), int(bool(force)), C.byref(p)))
+    return {k: getattr(p, k) for k, _ in BatchPlan._fields_}
 
 
 class Batch:

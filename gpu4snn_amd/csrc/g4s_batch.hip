@@ -173,9 +173,35 @@ int check_slot(const g4s_batch *b, int32_t slot)
     return G4S_OK;
 }
 
-int slice_len_for(int64_t n, int32_t nsyn, int32_t requested)
+// Tuned slice lengths per bucket (floor(log2 N), Nsyn, accum), filled by
+// g4s_batch_tune for the life of the process.
+struct TunedLen {
+    int slice_len;
+    float us_per_inst_step;
+    float rate_hz;
+};
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int>, TunedLen> g_tuned;
+
+int ilog2i(int64_t x)
 {
-    return requested > 0 ? requested : default_slice_len(n, nsyn);
+    int r = -1;
+    while (x > 0) {
+        x >>= 1;
+        ++r;
+    }
+    return r;
+}
+
+int slice_len_for(int64_t n, int32_t nsyn, int32_t requested, int accum)
+{
+    if (requested > 0) return requested;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tuned.find(std::make_tuple(ilog2i(n), (int)nsyn, accum));
+        if (it != g_tuned.end()) return (int)std::min<int64_t>(it->second.slice_len, (n + 63) / 64 * 64);
+    }
+    return default_slice_len(n, nsyn);
 }
 
 }  // namespace
@@ -186,7 +212,7 @@ int64_t g4s_batch_footprint(const g4s_spec *spec, int32_t chunk, int32_t slice_l
 {
     if (validate_spec(spec)) return -1;
     const int64_t n = spec->n;
-    const int L = slice_len_for(n, spec->nsyn, slice_len);
+    const int L = slice_len_for(n, spec->nsyn, slice_len, -1);
     const int64_t P = (n + L - 1) / L;
     const int64_t words = (n + 31) / 32;
     const int cs = chunk > 0 ? chunk : kDefaultChunk;
@@ -228,7 +254,7 @@ int g4s_batch_add(g4s_batch *b, const g4s_spec *spec, int64_t steps, int keep_ra
     if (steps < 0 || steps > 0x7FFFFFFFLL) return fail(G4S_ERR_ARG, "steps out of range");
     if (spec->nsyn > 65535) return fail(G4S_ERR_UNSUPPORTED, "batched instances need Nsyn < 65536");
     const int64_t n = spec->n;
-    const int L = slice_len_for(n, spec->nsyn, b->opts.slice_len);
+    const int L = slice_len_for(n, spec->nsyn, b->opts.slice_len, b->accum);
     const bool det = b->accum == G4S_ACCUM_DET;
     if (batch_lds(L, det) > 64 * 1024)
         return fail(G4S_ERR_UNSUPPORTED, "slice length %d needs %zu bytes of LDS", L, batch_lds(L, det));
@@ -586,6 +612,87 @@ int g4s_batch_timing(const g4s_batch *b, g4s_timing *out)
 {
     if (!b || !out) return fail(G4S_ERR_ARG, "NULL argument");
     *out = b->timing;
+    return G4S_OK;
+}
+
+int g4s_batch_tune(int accum, const g4s_spec *rep, int32_t trial_steps, int force, g4s_batch_plan *out)
+{
+    int rc = validate_spec(rep);
+    if (rc) return rc;
+    if (!out) return fail(G4S_ERR_ARG, "out is NULL");
+    if (accum != G4S_ACCUM_ATOMIC && accum != G4S_ACCUM_DET) return fail(G4S_ERR_ARG, "unknown accum %d", accum);
+    if (trial_steps < 10) trial_steps = 100;
+    const auto key = std::make_tuple(ilog2i(rep->n), (int)rep->nsyn, accum);
+    if (!force) {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end()) {
+            out->slice_len = it->second.slice_len;
+            out->us_per_inst_step = it->second.us_per_inst_step;
+            out->rate_hz = it->second.rate_hz;
+            out->copies = 0;
+            return G4S_OK;
+        }
+    }
+    // Each candidate runs enough copies (different seeds) to keep the device
+    // full -- about 2000 workgroups per step, within 4 GiB -- so that
+    // throughput, not the latency of one instance, decides.
+    const int64_t n = rep->n;
+    const int warm = std::max(20, trial_steps / 4);
+    TunedLen best{0, INFINITY, 0.0f};
+    int copies = 0;
+    for (int L : {128, 256, 512, 1024, 2048}) {
+        if (L > (n + 63) / 64 * 64 * 2 && L > 128) continue;  // one slice already covers N
+        if (batch_lds(L, accum == G4S_ACCUM_DET) > 64 * 1024) continue;
+        const int64_t P = (n + L - 1) / L;
+        const int64_t fp = g4s_batch_footprint(rep, trial_steps + warm, L);
+        copies = (int)std::max<int64_t>(4, std::min<int64_t>({(2000 + P - 1) / P, 1024, ((int64_t)4 << 30) / fp}));
+        g4s_batch_opts o{};
+        o.chunk = trial_steps + warm;
+        o.slice_len = L;
+        g4s_batch *b = nullptr;
+        if ((rc = g4s_batch_create(accum, &o, &b))) return rc;
+        for (int k = 0; k < copies && rc == G4S_OK; ++k) {
+            g4s_spec sp = *rep;
+            sp.seed = rep->seed + 0x9E3779B97F4A7C15ull * (uint64_t)(k + 1);
+            int32_t slot;
+            rc = g4s_batch_add(b, &sp, warm + trial_steps, 0, &slot);
+        }
+        // one chunk of warm-up, one timed chunk
+        if (rc == G4S_OK) {
+            b->chunk = warm;
+            int32_t act = 0;
+            rc = g4s_batch_step(b, &act);
+        }
+        g4s_timing tm{};
+        if (rc == G4S_OK) {
+            b->chunk = trial_steps;
+            int32_t act = 0;
+            rc = g4s_batch_step(b, &act);
+            tm = b->timing;
+        }
+        g4s_batch_destroy(b);
+        if (rc == G4S_ERR_UNSUPPORTED || rc == G4S_ERR_MEM) {
+            rc = G4S_OK;
+            continue;
+        }
+        if (rc) return rc;
+        const float us = (float)(tm.device_ms * 1000.0 / ((double)trial_steps * copies));
+        if (us < best.us_per_inst_step) {
+            best.slice_len = L;
+            best.us_per_inst_step = us;
+            best.rate_hz = (float)((double)tm.spikes / ((double)tm.steps * (double)n) * 1000.0);
+        }
+    }
+    if (best.slice_len == 0) return fail(G4S_ERR_UNSUPPORTED, "no slice length ran for N=%lld", (long long)n);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned[key] = best;
+    }
+    out->slice_len = best.slice_len;
+    out->us_per_inst_step = best.us_per_inst_step;
+    out->rate_hz = best.rate_hz;
+    out->copies = copies;
     return G4S_OK;
 }
 

@@ -709,9 +709,24 @@ __global__ void __launch_bounds__(1024) k_slice_step(DevNet d, int rel)
     const int p = slice_of_block((int)blockIdx.x, (int)gridDim.x);
     const int lo = p * d.L;
     const int len = min(d.L, d.n - lo);
-    slice_accumulate<kDet, kEntriesSingle, kSpikesSingle>(d, t, p, lo, len, acc, (uint32_t *)(dyn + d.L));
     const int B = (int)blockDim.x;
     const int ntile = (len + B - 1) / B;
+    // The neurons' state and thalamic draws do not depend on the spikes:
+    // load and compute them first so that their latency overlaps the
+    // accumulation's dependent loads.
+    float v[R], u[R], a[R], b[R], c[R], dd[R], I0[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int off = k * B + (int)threadIdx.x;
+        v[k] = u[k] = a[k] = b[k] = c[k] = dd[k] = I0[k] = 0.0f;
+        if (k < ntile && off < len) {
+            const int j = lo + off;
+            v[k] = d.v[j]; u[k] = d.u[j];
+            a[k] = d.a[j]; b[k] = d.b[j]; c[k] = d.c[j]; dd[k] = d.d[j];
+            I0[k] = ((j < d.n_exc) ? d.ge : d.gi) * thalamic(d.seed, (uint32_t)t, (uint32_t)j);
+        }
+    }
+    slice_accumulate<kDet, kEntriesSingle, kSpikesSingle>(d, t, p, lo, len, acc, (uint32_t *)(dyn + d.L));
     uint64_t masks[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
@@ -720,7 +735,14 @@ __global__ void __launch_bounds__(1024) k_slice_step(DevNet d, int rel)
         const int off = k * B + (int)threadIdx.x;
         const int j = lo + off;
         bool spk = false;
-        if (off < len) spk = neuron_step(d, t, j, acc[off], last);
+        if (off < len) {
+            const float I = I0[k] + acc[off];  // (g * z) + input, as neuron_step
+            spk = izh_step(v[k], u[k], a[k], b[k], c[k], dd[k], I);
+            d.v[j] = v[k];
+            d.u[j] = u[k];
+            if (spk) d.nospks[j] += 1;
+            if (last) d.lastI[j] = I;
+        }
         masks[k] = emit_bits(d, t, rel, lo + k * B + ((int)threadIdx.x & ~(kWave - 1)), spk, lo + len);
     }
     slice_count<R>(d, rel, p, ntile, masks, &s_cnt);

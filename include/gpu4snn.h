@@ -191,7 +191,8 @@ typedef struct g4s_batch g4s_batch;
 
 typedef struct g4s_batch_opts {
     int32_t chunk;       /* steps per chunk, 0 = 100                                */
-    int32_t slice_len;   /* neurons per slice (multiple of 64), 0 = min(512, N)     */
+    int32_t slice_len;   /* neurons per slice (multiple of 64), 0 = the bucket's
+                            g4s_batch_tune choice, else min(512, N)              */
     int32_t graph;       /* schedule 0: >= 0 replays the chunk's launches from a HIP
                             graph, < 0 launches them directly                       */
     int32_t schedule;    /* 0: one launch per step over all instances;
@@ -242,6 +243,21 @@ int g4s_batch_remove(g4s_batch *b, int32_t slot);
  * launches of the batched step kernel, grid = its largest grid). */
 int g4s_batch_timing(const g4s_batch *b, g4s_timing *out);
 int g4s_batch_destroy(g4s_batch *b);
+
+/* Batch autotuner (no reference counterpart).  Times the slice lengths 128
+ * .. 2048 on a batch of copies of `rep` (different seeds; enough copies to
+ * fill the device, so throughput decides) for trial_steps steps after a
+ * warm-up, and records the fastest for the bucket (floor(log2 N), Nsyn,
+ * accum) for the life of the process: g4s_batch_add then uses it for every
+ * instance in that bucket when the batch's slice_len is 0.  A recorded
+ * bucket is returned without timing unless `force`. */
+typedef struct g4s_batch_plan {
+    int32_t slice_len;         /* chosen neurons per slice                  */
+    int32_t copies;            /* instances timed (0: cached plan)           */
+    float   us_per_inst_step;  /* device time per instance and step          */
+    float   rate_hz;           /* firing rate seen while timing              */
+} g4s_batch_plan;
+int g4s_batch_tune(int accum, const g4s_spec *rep, int32_t trial_steps, int force, g4s_batch_plan *out);
 
 /* ------------------------------------------------------------------------
  * Serving (no reference counterpart).  A server owns one batch and a worker

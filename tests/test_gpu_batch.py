@@ -175,3 +175,18 @@ def test_batch_counts_only_and_invalid(g4s):
             b.info(99)
         with pytest.raises(g4s.G4SError):
             b.add(10, 10, 1, 5)  # Nsyn >= N
+
+
+def test_batch_tune_sets_bucket_plan(g4s, oracle):
+    """g4s_batch_tune picks a slice length for the (N, Nsyn) bucket; later
+    instances of the bucket use it, with unchanged det results."""
+    plan = g4s.batch_tune(3000, 300, accum="det", trial_steps=40, force=True)
+    assert plan["slice_len"] in (128, 256, 512, 1024, 2048) and plan["copies"] >= 4
+    assert plan["us_per_inst_step"] > 0
+    again = g4s.batch_tune(3000, 300, accum="det", trial_steps=40)
+    assert again["slice_len"] == plan["slice_len"] and again["copies"] == 0  # cached
+    with g4s.Batch("det", chunk=50) as b:
+        s = b.add(3500, 300, 77, 120)  # same bucket: floor(log2 N) = 11
+        b.run()
+        assert b.info(s)["slice_len"] == min(plan["slice_len"], 3520)
+        check_instance(g4s, oracle, b, s, 3500, 300, 77, 1.0, 120)
