@@ -100,7 +100,8 @@ def bench_batch(g, args):
         print(f"tuned {len(tuned)} buckets in {time.perf_counter() - t_tune:.1f} s: "
               f"{sorted(tuned.items())}", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    b = g.Batch(args.accum, chunk=args.chunk, slice_len=args.slice_len, schedule=args.schedule)
+    b = g.Batch(args.accum, chunk=args.chunk, slice_len=args.slice_len, schedule=args.schedule,
+                block=args.batch_block)
     slots = [b.add(n, ns, sd, total, thal_scale=sc, keep_raster=False) for (n, ns, sd, sc) in specs]
     print(f"batch of {len(slots)} built in {time.perf_counter() - t0:.1f} s, "
           f"{sum(n for n, _, _, _ in specs)} neurons, {sum(n * ns for n, ns, _, _ in specs) / 1e9:.2f}e9 synapses",
@@ -149,7 +150,8 @@ def bench_batch(g, args):
                    "accum": args.accum, "chunk": args.chunk,
                    "slice_len": args.slice_len or ("tuned per bucket" if tuned else "min(512, N)"),
                    "tuned": {f"log2N={k[0]},Nsyn={k[1]}": v for k, v in sorted(tuned.items())} or None,
-                   "schedule": args.schedule, "neurons": sum(x[0] for x in specs),
+                   "schedule": args.schedule, "block": args.batch_block or 256,
+                   "neurons": sum(x[0] for x in specs),
                    "synapses": sum(x[0] * x[1] for x in specs), "seed": args.seed},
         "events_per_s": events / wall,
         "plain": {"sim_s_per_s": p_sim / p_wall, "sample": len(idx), "scheme": "AB"},
@@ -292,6 +294,7 @@ def main():
     ap.add_argument("--chunk", type=int, default=100)
     ap.add_argument("--slice-len", type=int, default=0, help="batched: neurons per slice, 0 = per instance")
     ap.add_argument("--schedule", default="step", choices=["step", "queue"])
+    ap.add_argument("--batch-block", type=int, default=0, help="batched: threads per workgroup (256/512/1024)")
     ap.add_argument("--plain-sample", type=int, default=32)
     ap.add_argument("--tune", action="store_true", help="batched: tune the slice length per (N, Nsyn) bucket first")
     ap.add_argument("--no-extras", action="store_true",

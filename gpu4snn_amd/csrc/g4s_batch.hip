@@ -87,6 +87,7 @@ struct g4s_batch {
     int accum = G4S_ACCUM_ATOMIC;
     g4s_batch_opts opts{};
     int chunk = kDefaultChunk;
+    int block = kBlock;  // threads per workgroup of the per-step kernel
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<Inst> inst;
@@ -231,12 +232,16 @@ int g4s_batch_create(int accum, const g4s_batch_opts *opts, g4s_batch **out)
     if (o.slice_len < 0 || o.slice_len % 64 != 0 || o.slice_len > kMaxSliceLen)
         return fail(G4S_ERR_ARG, "slice_len must be a multiple of 64 <= %d", kMaxSliceLen);
     if (o.schedule < 0 || o.schedule > 1 || o.budget < 0) return fail(G4S_ERR_ARG, "bad options");
+    if (o.block != 0 && o.block != 256 && o.block != 512 && o.block != 1024)
+        return fail(G4S_ERR_ARG, "block must be 0, 256, 512 or 1024");
+    if (o.schedule == 1 && o.block > 256) return fail(G4S_ERR_ARG, "the queue schedule runs 256-thread workgroups");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(G4S_ERR_HIP, "no HIP device");
     g4s_batch *b = new g4s_batch();
     b->accum = accum;
     b->opts = o;
     b->chunk = o.chunk > 0 ? o.chunk : kDefaultChunk;
+    b->block = o.block > 0 ? o.block : kBlock;
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
         g4s_batch_destroy(b);
@@ -256,8 +261,8 @@ int g4s_batch_add(g4s_batch *b, const g4s_spec *spec, int64_t steps, int keep_ra
     const int64_t n = spec->n;
     const int L = slice_len_for(n, spec->nsyn, b->opts.slice_len, b->accum);
     const bool det = b->accum == G4S_ACCUM_DET;
-    if (batch_lds(L, det) > 64 * 1024)
-        return fail(G4S_ERR_UNSUPPORTED, "slice length %d needs %zu bytes of LDS", L, batch_lds(L, det));
+    if (batch_lds(L, det, b->block) > 64 * 1024)
+        return fail(G4S_ERR_UNSUPPORTED, "slice length %d needs %zu bytes of LDS", L, batch_lds(L, det, b->block));
     const int64_t P = (n + L - 1) / L;
     const int64_t fp = g4s_batch_footprint(spec, b->chunk, L);
     if (b->opts.budget > 0 && b->bytes_in_use + fp > b->opts.budget)
@@ -418,8 +423,8 @@ int g4s_batch_step(g4s_batch *b, int32_t *active)
     HIP_OK(hipMemcpyAsync(b->d_work, b->h_work, (size_t)grid * sizeof(uint2), hipMemcpyHostToDevice, b->stream));
     HIP_OK(hipMemcpyAsync(b->d_slots, b->h_slots, act.size() * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
 
-    const size_t lds = batch_lds(Lmax, det);
-    const Launch L{grid, kBlock, b->stream};
+    const size_t lds = batch_lds(Lmax, det, b->block);
+    const Launch L{grid, b->block, b->stream};
     int launch_grid = grid;
     if (b->opts.schedule == 1) {
         // queue: the first step of every instance, costlier instances first

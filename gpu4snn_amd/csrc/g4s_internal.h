@@ -107,12 +107,12 @@ int skl_slice_blocks_per_cu(int L, int block, bool det);
 // Dynamic LDS bytes of the slice kernels (single network at `block`
 // threads; batched at 256).
 size_t slice_lds(int L, int block, bool det);
-size_t batch_lds(int L, bool det);
+size_t batch_lds(int L, bool det, int block = 256);
 hipError_t launch_pull_thread(const DevNet &d, int t, int rel, int last, const Launch &L);
 hipError_t launch_pull_wave(const DevNet &d, int t, int rel, int last, const Launch &L);
 hipError_t launch_ap_det(const DevNet &d, int t, int rel, int last, const Launch &L);
 // Batched step `rel` of a chunk: one workgroup per work unit, grid = units
-// (padded entries exit).  block = 256.
+// (padded entries exit).  L.block = 256, 512 or 1024.
 hipError_t launch_batch_step(const DevNet *descs, const uint2 *work, int grid, int rel, bool det, size_t lds,
                              const Launch &L);
 // One persistent launch running a whole chunk from the device work queue

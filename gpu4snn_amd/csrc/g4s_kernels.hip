@@ -758,9 +758,9 @@ __device__ __forceinline__ void batch_unit(const DevNet &d, int rel, int p, floa
     const int lo = p * d.L;
     const int len = min(d.L, d.n - lo);
     float *acc = dyn;
-    slice_accumulate<kDet, kEntriesBatch, spikes_batch<kDet>()>(d, t, p, lo, len, acc, (uint32_t *)(dyn + d.L));
     const int B = (int)blockDim.x;
     const int tid = (int)threadIdx.x;
+    slice_accumulate<kDet, kEntriesBatch, spikes_batch<kDet>()>(d, t, p, lo, len, acc, (uint32_t *)(dyn + d.L));
     const bool last = t == d.last_t;
     if (tid == 0) *s_cnt = 0;
     __syncthreads();
@@ -780,9 +780,9 @@ __device__ __forceinline__ void batch_unit(const DevNet &d, int rel, int p, floa
 // One launch per step: workgroup b takes work unit work[b] = {slot, p} at
 // step rel; the host orders the units so that each XCD holds whole
 // instances.  Instances whose chunk ends before rel do nothing.
-template <bool kDet>
-__global__ void __launch_bounds__(256, 6) k_batch_step(const DevNet *__restrict__ descs, const uint2 *__restrict__ work,
-                                                    int rel)
+template <bool kDet, int kB>
+__global__ void __launch_bounds__(kB, kB == 256 ? (kDet ? 6 : 8) : 1) k_batch_step(const DevNet *__restrict__ descs,
+                                                                     const uint2 *__restrict__ work, int rel)
 {
     extern __shared__ float dyn[];
     __shared__ int s_cnt;
@@ -1348,9 +1348,9 @@ int skl_slice_blocks_per_cu(int L, int block, bool det)
 }
 
 size_t slice_lds(int L, int block, bool det) { return slice_lds_bytes(L, block, det, kEntriesSingle, kSpikesSingle); }
-size_t batch_lds(int L, bool det)
+size_t batch_lds(int L, bool det, int block)
 {
-    return slice_lds_bytes(L, 256, det, kEntriesBatch, det ? spikes_batch<true>() : spikes_batch<false>());
+    return slice_lds_bytes(L, block, det, kEntriesBatch, det ? spikes_batch<true>() : spikes_batch<false>());
 }
 
 hipError_t launch_pull_thread(const DevNet &d, int t, int rel, int last, const Launch &L)
@@ -1391,8 +1391,15 @@ hipError_t launch_batch_step(const DevNet *descs, const uint2 *work, int grid, i
 {
     Launch Lb = L;
     Lb.grid = grid;
-    if (det) return plain_launch(k_batch_step<true>, Lb, (uint32_t)lds, descs, work, rel);
-    return plain_launch(k_batch_step<false>, Lb, (uint32_t)lds, descs, work, rel);
+    switch (L.block * 2 + (det ? 1 : 0)) {
+    case 512: return plain_launch(k_batch_step<false, 256>, Lb, (uint32_t)lds, descs, work, rel);
+    case 513: return plain_launch(k_batch_step<true, 256>, Lb, (uint32_t)lds, descs, work, rel);
+    case 1024: return plain_launch(k_batch_step<false, 512>, Lb, (uint32_t)lds, descs, work, rel);
+    case 1025: return plain_launch(k_batch_step<true, 512>, Lb, (uint32_t)lds, descs, work, rel);
+    case 2048: return plain_launch(k_batch_step<false, 1024>, Lb, (uint32_t)lds, descs, work, rel);
+    case 2049: return plain_launch(k_batch_step<true, 1024>, Lb, (uint32_t)lds, descs, work, rel);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_batch_counts(const DevNet *descs, const int32_t *slots, int nslots, int cs, int32_t *counts,

@@ -199,6 +199,8 @@ typedef struct g4s_batch_opts {
                             1: one persistent launch per chunk, work units queued on
                             the device as each instance's previous step completes */
     int64_t budget;      /* device bytes the batch may hold, 0 = no limit            */
+    int32_t block;       /* schedule 0: threads per workgroup, 256 (0), 512 or 1024   */
+    int32_t reserved;    /* must be 0                                                */
 } g4s_batch_opts;
 
 typedef struct g4s_inst_info {
