@@ -48,7 +48,8 @@ __device__ __forceinline__ void raise_error(const DevNet &d)
 
 // One Izhikevich step of neuron j at step t with synaptic input `isyn`
 // (cu:358-407).  Writes v, u, nospks and, on the run's last step, lastI.
-__device__ __forceinline__ bool neuron_step(const DevNet &d, int t, int j, float isyn, bool last)
+template <class Net>
+__device__ __forceinline__ bool neuron_step(const Net &d, int t, int j, float isyn, bool last)
 {
     const float z = thalamic(d.seed, (uint32_t)t, (uint32_t)j);
     const float g = (j < d.n_exc) ? d.ge : d.gi;
@@ -381,12 +382,13 @@ __device__ __forceinline__ int slice_of_block(int b, int g)
 // Bitmap and raster words of one wave covering [wave_base, wave_base+64)
 // of a slice kernel; returns the wave's spike mask.  Waves starting at or
 // beyond `limit` write nothing.
-__device__ __forceinline__ uint64_t emit_bits(const DevNet &d, int t, int rel, int wave_base, bool spk, int limit)
+template <class Net>
+__device__ __forceinline__ uint64_t emit_bits(const Net &d, int t, int rel, int wave_base, bool spk, int limit)
 {
     const uint64_t mask = __ballot(spk);
     if (wave_base >= limit) return mask;  // uniform over the wave
     const int lane = lane_id();
-    uint32_t *bm = (t & 1) ? d.bm1 : d.bm0;
+    auto bm = (t & 1) ? d.bm1 : d.bm0;
     const int w = (wave_base >> 5) + lane;
     if (lane < 2 && w < d.words) {
         const uint32_t bits = lane ? (uint32_t)(mask >> 32) : (uint32_t)mask;
@@ -467,8 +469,16 @@ __host__ __device__ inline size_t slice_lds_bytes(int L, int block, bool det, in
 
 // Entry f of the current pass (concatenated segments): binary search for
 // the last spike a with segp[a] <= f, then the table entry.
-__device__ __forceinline__ uint2 pass_entry(const DevNet &d, const uint32_t *ids, const uint32_t *seg0,
-                                            const int *segp, int m, int f)
+// One {target, weight bits} table entry (generic or global pointer).
+__device__ __forceinline__ uint2 entry_at(const uint2 *p, size_t i) { return p[i]; }
+__device__ __forceinline__ uint2 entry_at(__attribute__((address_space(1))) const unsigned long long *p, size_t i)
+{
+    const unsigned long long x = p[i];
+    return make_uint2((uint32_t)x, (uint32_t)(x >> 32));
+}
+
+// Index of the last spike a with segp[a] <= f (segp ascending, segp[0] = 0).
+__device__ __forceinline__ int segment_of(const int *segp, int m, int f)
 {
     int a = 0, b = m - 1;
     while (a < b) {
@@ -476,7 +486,7 @@ __device__ __forceinline__ uint2 pass_entry(const DevNet &d, const uint32_t *ids
         if (segp[mid] <= f) a = mid;
         else b = mid - 1;
     }
-    return d.syn[(size_t)ids[a] * (size_t)d.nsyn + (size_t)seg0[a] + (size_t)(f - segp[a])];
+    return a;
 }
 
 // acc[0..len) <- input of slice p from the rows that spiked at step t-1.
@@ -494,8 +504,8 @@ __device__ __forceinline__ uint2 pass_entry(const DevNet &d, const uint32_t *ids
 // (source, target) order; each window of them is counting-sorted by target
 // in LDS (count, scan, place) and each target then adds its entries in key
 // order, so the sum never depends on the order of the LDS atomics.
-template <bool kDet, int kEPT, int kSPT>
-__device__ __forceinline__ void slice_accumulate(const DevNet &d, int t, int p, int lo, int len, float *acc,
+template <bool kDet, int kEPT, int kSPT, class Net>
+__device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int lo, int len, float *acc,
                                                  uint32_t *ws)
 {
     const int B = (int)blockDim.x;
@@ -520,7 +530,7 @@ __device__ __forceinline__ void slice_accumulate(const DevNet &d, int t, int p, 
         __syncthreads();
         return;
     }
-    const uint32_t *bm = ((t - 1) & 1) ? d.bm1 : d.bm0;
+    const auto bm = ((t - 1) & 1) ? d.bm1 : d.bm0;
     const int W = d.words;
     const int per = (W + B - 1) / B;
     const int wb = min(tid * per, W), we = min(wb + per, W);
@@ -573,7 +583,7 @@ __device__ __forceinline__ void slice_accumulate(const DevNet &d, int t, int p, 
             const int i = tid * kSPT + k;
             lens[k] = 0;
             if (i < m) {
-                const uint16_t *o = d.soff + (size_t)ids[i] * stride + (size_t)p;
+                const auto o = d.soff + (size_t)ids[i] * stride + (size_t)p;
                 const uint32_t e0 = o[0], e1 = o[1];
                 seg0[i] = e0;
                 lens[k] = (int)(e1 - e0);
@@ -591,14 +601,20 @@ __device__ __forceinline__ void slice_accumulate(const DevNet &d, int t, int p, 
         if (tid == 0) segp[m] = E;
         __syncthreads();
         // 3. entries, thread per synapse, windows of kEPT*B entries with all
-        // of a window's loads in flight
+        // of a window's loads in flight.  Lanes take consecutive entries
+        // (f = f0 + k*B + tid): the loads coalesce, and a target's
+        // contributions reach the LDS atomics nearly in ascending source
+        // order, which keeps atomic mode on the ordered sum for many steps.
         for (int f0 = 0; f0 < E; f0 += Wd) {
             uint2 x[kEPT];
 #pragma unroll
             for (int k = 0; k < kEPT; ++k) {
                 const int f = f0 + k * B + tid;
                 x[k] = make_uint2(0xFFFFFFFFu, 0u);
-                if (f < E) x[k] = pass_entry(d, ids, seg0, segp, m, f);
+                if (f < E) {
+                    const int sa = segment_of(segp, m, f);
+                    x[k] = entry_at(d.syn, (size_t)ids[sa] * (size_t)d.nsyn + (size_t)seg0[sa] + (size_t)(f - segp[sa]));
+                }
             }
             if (!kDet) {
 #pragma unroll
@@ -748,11 +764,47 @@ __global__ void __launch_bounds__(1024) k_slice_step(DevNet d, int rel)
     slice_count<R>(d, rel, p, ntile, masks, &s_cnt);
 }
 
+// Descriptor of a batched instance as the batched kernels use it: the
+// fields they read, with pointers in the global address space.  The table
+// in device memory holds DevNets; pointers loaded from memory are generic,
+// and generic (flat) accesses would also wait on LDS traffic.
+#define G4S_GLOBAL __attribute__((address_space(1)))
+struct BNet {
+    int32_t n, n_exc, nsyn, words;
+    uint64_t seed;
+    float ge, gi;
+    int32_t P, L, t_base, rel_end, last_t;
+    G4S_GLOBAL float *v, *u, *a, *b, *c, *d, *lastI;
+    G4S_GLOBAL int32_t *nospks, *bcnt;
+    G4S_GLOBAL const unsigned long long *syn;  // {target, weight bits} as one word
+    G4S_GLOBAL const uint16_t *soff;
+    G4S_GLOBAL uint32_t *bm0, *bm1, *raster;
+};
+
+__device__ __forceinline__ BNet bnet_of(const DevNet &x)
+{
+    BNet r;
+    r.n = x.n; r.n_exc = x.n_exc; r.nsyn = x.nsyn; r.words = x.words;
+    r.seed = x.seed; r.ge = x.ge; r.gi = x.gi;
+    r.P = x.P; r.L = x.L; r.t_base = x.t_base; r.rel_end = x.rel_end; r.last_t = x.last_t;
+    r.v = (G4S_GLOBAL float *)x.v; r.u = (G4S_GLOBAL float *)x.u;
+    r.a = (G4S_GLOBAL float *)x.a; r.b = (G4S_GLOBAL float *)x.b;
+    r.c = (G4S_GLOBAL float *)x.c; r.d = (G4S_GLOBAL float *)x.d;
+    r.lastI = (G4S_GLOBAL float *)x.lastI;
+    r.nospks = (G4S_GLOBAL int32_t *)x.nospks;
+    r.bcnt = (G4S_GLOBAL int32_t *)x.bcnt;
+    r.syn = (G4S_GLOBAL const unsigned long long *)x.syn;
+    r.soff = (G4S_GLOBAL const uint16_t *)x.soff;
+    r.bm0 = (G4S_GLOBAL uint32_t *)x.bm0; r.bm1 = (G4S_GLOBAL uint32_t *)x.bm1;
+    r.raster = (G4S_GLOBAL uint32_t *)x.raster;
+    return r;
+}
+
 // Batched form.  A work unit is slice p of one instance at chunk step rel:
 // it accumulates that slice's input from the instance's previous step,
 // updates the slice's neurons and records its spike count.
 template <bool kDet>
-__device__ __forceinline__ void batch_unit(const DevNet &d, int rel, int p, float *dyn, int *s_cnt)
+__device__ __forceinline__ void batch_unit(const BNet &d, int rel, int p, float *dyn, int *s_cnt)
 {
     const int t = d.t_base + rel;
     const int lo = p * d.L;
@@ -790,7 +842,7 @@ __global__ void __launch_bounds__(kB, kB == 256 ? (kDet ? 6 : 8) : 1) k_batch_st
     if (wu.x == kNoWork) return;
     // a private copy: the kernel's stores could otherwise alias the table,
     // forcing a reload of every field after each store
-    const DevNet d = descs[wu.x];
+    const BNet d = bnet_of(descs[wu.x]);
     if (rel >= d.rel_end) return;
     batch_unit<kDet>(d, rel, (int)wu.y, dyn, &s_cnt);
 }
@@ -843,7 +895,7 @@ __global__ void __launch_bounds__(256, 6) k_batch_chunk(const DevNet *__restrict
         const uint32_t slot = (uint32_t)u;
         const int rel = (int)(u >> 48);
         const int p = (int)((u >> 32) & 0xFFFFu);
-        const DevNet d = descs[slot];
+        const BNet d = bnet_of(descs[slot]);
         batch_unit<kDet>(d, rel, p, dyn, &s_cnt);
         // release this slice; the last one of the step publishes the next
         __syncthreads();
