@@ -11,7 +11,9 @@
 #include "g4s_oracle.h"
 
 #include <math.h>
+#include <pthread.h>
 #include <stdlib.h>
+#include <unistd.h>
 #include <string.h>
 
 /* ------------------------------------------------------------------ */
@@ -472,4 +474,98 @@ int orc_simulate_f64_forced_ranges(const orc_net *net, int32_t t0, int32_t steps
     }
     free(isyn);
     return 0;
+}
+
+/* ------------------------------------------------------------------------
+ * Multithreaded orc_simulate (all-cores CPU baseline).
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    const orc_net *net;
+    int32_t t0, steps, nthreads, k;
+    float *v, *u, *isyn, *lastI;
+    int32_t *nospks, *counts, *part;
+    uint8_t *spk, *raster;
+    int64_t *list;   /* spiking ids of the step, ascending */
+    int64_t *nlist;
+    pthread_barrier_t *bar;
+} orc_mt_arg;
+
+static void *orc_mt_worker(void *p)
+{
+    orc_mt_arg *a = (orc_mt_arg *)p;
+    const orc_net *net = a->net;
+    const int64_t n = net->n;
+    const int64_t per = (n + a->nthreads - 1) / a->nthreads;
+    const int64_t lo = (int64_t)a->k * per, hi = lo + per < n ? lo + per : n;
+    for (int32_t s = 0; s < a->steps; ++s) {
+        const uint32_t t = (uint32_t)(a->t0 + s);
+        int32_t cnt = 0;
+        for (int64_t i = lo; i < hi; ++i) {
+            float z = orc_thalamic(net->seed, t, (uint32_t)i);
+            float g = (i < net->n_exc) ? net->ge : net->gi;
+            float I = g * z;
+            I = I + a->isyn[i];
+            a->isyn[i] = 0.0f;
+            if (a->lastI && s == a->steps - 1) a->lastI[i] = I;
+            a->spk[i] = (uint8_t)izh_step(&a->v[i], &a->u[i], net->a[i], net->b[i], net->c[i], net->d[i], I);
+            if (a->spk[i]) { a->nospks[i] += 1; cnt += 1; }
+        }
+        a->part[a->k] = cnt;
+        pthread_barrier_wait(a->bar);
+        if (a->k == 0) {  /* ascending spike list and the step's outputs */
+            int64_t m = 0;
+            int32_t total = 0;
+            for (int64_t i = 0; i < n; ++i)
+                if (a->spk[i]) a->list[m++] = i;
+            for (int32_t q = 0; q < a->nthreads; ++q) total += a->part[q];
+            *a->nlist = m;
+            if (a->raster) memcpy(a->raster + (size_t)s * (size_t)n, a->spk, (size_t)n);
+            if (a->counts) a->counts[s] = total;
+        }
+        pthread_barrier_wait(a->bar);
+        /* rows that spiked, ascending, into this thread's targets */
+        const int64_t m = *a->nlist;
+        for (int64_t q = 0; q < m; ++q) {
+            const int64_t i = a->list[q];
+            const uint32_t *row = net->post + (size_t)i * (size_t)net->nsyn;
+            const float *wr = net->w + (size_t)i * (size_t)net->nsyn;
+            int32_t e = orc_lower_bound(row, net->nsyn, (uint32_t)lo);
+            for (; e < net->nsyn && (int64_t)row[e] < hi; ++e) a->isyn[row[e]] = a->isyn[row[e]] + wr[e];
+        }
+        pthread_barrier_wait(a->bar);
+    }
+    return NULL;
+}
+
+int orc_simulate_mt(const orc_net *net, int32_t t0, int32_t steps,
+                    float *v, float *u, float *isyn, int32_t *nospks,
+                    uint8_t *raster, int32_t *counts, float *lastI, int32_t threads)
+{
+    if (threads <= 0) threads = (int32_t)sysconf(_SC_NPROCESSORS_ONLN);
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    if ((int64_t)threads > net->n) threads = (int32_t)net->n;
+    uint8_t *spk = (uint8_t *)malloc((size_t)net->n);
+    int64_t *list = (int64_t *)malloc(sizeof(int64_t) * (size_t)net->n);
+    int32_t *part = (int32_t *)calloc((size_t)threads, sizeof(int32_t));
+    orc_mt_arg *args = (orc_mt_arg *)calloc((size_t)threads, sizeof(orc_mt_arg));
+    pthread_t *th = (pthread_t *)calloc((size_t)threads, sizeof(pthread_t));
+    int64_t nlist = 0;
+    pthread_barrier_t bar;
+    int rc = 0;
+    if (!spk || !list || !part || !args || !th || pthread_barrier_init(&bar, NULL, (unsigned)threads) != 0) {
+        rc = -2;
+    } else {
+        for (int32_t k = 0; k < threads; ++k) {
+            orc_mt_arg a = {net, t0, steps, threads, k, v, u, isyn, lastI, nospks, counts, part, spk, raster,
+                            list, &nlist, &bar};
+            args[k] = a;
+        }
+        for (int32_t k = 1; k < threads; ++k) pthread_create(&th[k], NULL, orc_mt_worker, &args[k]);
+        orc_mt_worker(&args[0]);
+        for (int32_t k = 1; k < threads; ++k) pthread_join(th[k], NULL);
+        pthread_barrier_destroy(&bar);
+    }
+    free(spk); free(list); free(part); free(args); free(th);
+    return rc;
 }

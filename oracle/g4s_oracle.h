@@ -99,6 +99,16 @@ int orc_simulate(const orc_net *net, int32_t t0, int32_t steps,
                  float *v, float *u, float *isyn, int32_t *nospks,
                  uint8_t *raster, int32_t *counts, float *lastI);
 
+/* orc_simulate on `threads` host threads (<= 0: one per core), bit-identical
+ * to it: each thread updates a range of neurons, then sums into its own
+ * range of targets the rows that spiked, in ascending presynaptic order
+ * (each row's part for the range found by binary search in the sorted
+ * row).  The all-cores CPU baseline.
+ * Pinned: equal to orc_simulate bit for bit (tests/test_oracle_model.py). */
+int orc_simulate_mt(const orc_net *net, int32_t t0, int32_t steps,
+                    float *v, float *u, float *isyn, int32_t *nospks,
+                    uint8_t *raster, int32_t *counts, float *lastI, int32_t threads);
+
 /* fp64 twin of orc_simulate with teacher forcing: presynaptic spikes are
  * taken from `forced_raster` (steps*n bytes, e.g. a GPU raster) instead of
  * being generated; only neurons listed in `sub` (nsub ids) are integrated.

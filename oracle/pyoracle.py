@@ -69,6 +69,8 @@ def lib():
         L.orc_simulate_f64_forced.argtypes = [C.POINTER(OrcNet), C.c_int32, C.c_int32, _u8p, _i64p,
                                               C.c_int64, _f64p, _f64p, C.c_void_p, C.c_void_p]
         L.orc_simulate_f64_forced.restype = C.c_int
+        L.orc_simulate_mt.argtypes = list(L.orc_simulate.argtypes) + [C.c_int32]
+        L.orc_simulate_mt.restype = C.c_int
         L.orc_simulate_f64_forced_ranges.argtypes = [C.POINTER(OrcNet), C.c_int32, C.c_int32, C.c_void_p, C.c_int64,
                                                      _i64p, _i64p, C.c_int32, _f64p, _f64p, C.c_void_p, C.c_void_p]
         L.orc_simulate_f64_forced_ranges.restype = C.c_int
@@ -165,12 +167,22 @@ class Network:
                       self.post.ctypes.data, self.w.ctypes.data, self.a.ctypes.data,
                       self.b.ctypes.data, self.c.ctypes.data, self.d.ctypes.data)
 
-    def run(self, steps, raster=True):
-        """Advance `steps` steps; returns (raster uint8[steps, n] or None, counts, lastI)."""
+    def run(self, steps, raster=True, threads=1):
+        """Advance `steps` steps; returns (raster uint8[steps, n] or None, counts, lastI).
+        threads != 1 runs the multithreaded form (same results bit for bit;
+        <= 0: one thread per core)."""
         r = np.zeros((steps, self.n), np.uint8) if raster else None
         counts = np.zeros(steps, np.int32)
         last = np.zeros(self.n, np.float32)
         cn = self._cnet()
+        if threads != 1:
+            rc = lib().orc_simulate_mt(C.byref(cn), self.t, steps, self.v, self.u, self.isyn, self.nospks,
+                                       r.ctypes.data if raster else None, counts.ctypes.data, last.ctypes.data,
+                                       int(threads))
+            if rc != 0:
+                raise RuntimeError(f"orc_simulate_mt failed ({rc})")
+            self.t += steps
+            return r, counts, last
         rc = lib().orc_simulate(C.byref(cn), self.t, steps, self.v, self.u, self.isyn, self.nospks,
                                 r.ctypes.data if raster else None, counts.ctypes.data, last.ctypes.data)
         if rc != 0:

@@ -223,3 +223,19 @@ def test_f64_forced_ranges_equals_list_form(oracle):
     tr2, own2 = f.run_f64_forced_ranges(words, ranges, 0, v0[sub], u0[sub])
     assert np.array_equal(tr1, tr2) and np.array_equal(own1, own2)
     assert np.array_equal(own1, r[:, sub])
+
+
+@pytest.mark.parametrize("threads", [2, 5, 0])
+def test_multithreaded_oracle_is_bit_identical(oracle, threads):
+    """The all-cores oracle equals the single-thread one bit for bit
+    (rasters, final v/u/I, spike counts), including thread ranges that do
+    not divide N."""
+    a = oracle.Network(3001, 400, seed=19, regime=2)
+    b = oracle.Network(3001, 400, seed=19, regime=2)
+    ra, ca, la = a.run(150)
+    rb, cb, lb = b.run(150, threads=threads)
+    assert np.array_equal(ra, rb) and np.array_equal(ca, cb)
+    assert np.array_equal(a.v.view(np.uint32), b.v.view(np.uint32))
+    assert np.array_equal(a.u.view(np.uint32), b.u.view(np.uint32))
+    assert np.array_equal(la.view(np.uint32), lb.view(np.uint32))
+    assert np.array_equal(a.nospks, b.nospks)
