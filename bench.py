@@ -327,8 +327,8 @@ def main():
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(independent networks shard by placement; see DESIGN.md). Run with --gpus 1.",
+        print("bench.py measures one GPU: there is no multi-GPU path (independent networks shard by "
+              "placing whole networks on GPUs, with no traffic between them). Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
 

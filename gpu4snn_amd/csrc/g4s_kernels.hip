@@ -445,30 +445,42 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int *s_tmp, int *tota
     return s_tmp[wave] + incl - v;
 }
 
-// LDS workspace of slice_accumulate beyond acc[L]: per spike of a pass its
-// row id, first segment index and the prefix of segment lengths, plus the
-// scan scratch.  A pass handles kSPT spikes per thread.  The
-// ordered (det) variant adds, per window of kDetPerThread*B entries, a
-// count/cursor and a start per target and each entry's key and weight.
 // Spikes per pass per thread and entry loads in flight per thread:
 // single-network slices (up to 1024 threads) and batched slices (256).
 constexpr int kSpikesSingle = 2, kEntriesSingle = 4;
 constexpr int kEntriesBatch = 8;
-// batched det slices keep 2 spikes per thread: their LDS then allows 6
-// workgroups per CU at 512-neuron slices (4 would leave 5)
+// batched det slices keep 2 spikes per thread, which keeps their LDS
+// within 7 workgroups per CU at 512-neuron slices
 template <bool kDet>
 constexpr int spikes_batch() { return kDet ? 2 : 4; }
-__host__ __device__ inline size_t slice_lds_bytes(int L, int block, bool det, int ept, int spt)
+// How an entry of a window finds its spike: an owner map (batched slices,
+// whose workgroups are issue-bound: a few instructions per entry) or a
+// binary search over the segment offsets, all of a thread's entries in
+// lockstep (single-network slices, one latency-bound workgroup per CU: no
+// extra barriers).
+constexpr bool kOwnerSingle = false, kOwnerBatch = true;
+
+// LDS workspace of slice_accumulate.  acc[L] (floats, padded to 16 bytes),
+// then, for a pass of cap = kSPT*B spikes and windows of Wd = kEPT*B
+// entries:
+//   base   u64 x cap   per spike: table index of its segment's first entry
+//                      minus the segment's offset in the pass (the pass's
+//                      spike ids are gathered in its first half)
+//   scan   int x (B/64 + 1), padded to 16 bytes
+//   search lookup: segment offsets int x cap, padded to 16 bytes
+//   atomic, owner lookup: own u16 x Wd, twice (maps of even and odd windows)
+//   det:    w f32 x Wd | keys u16 x Wd (the owner map too) | cursor, start int x L
+__host__ __device__ inline int acc_floats(int L) { return (L + 3) & ~3; }
+__host__ __device__ inline size_t scan_bytes(int block) { return ((size_t)(block / 64 + 1) * 4 + 15) & ~(size_t)15; }
+__host__ __device__ inline size_t slice_lds_bytes(int L, int block, bool det, int ept, int spt, bool owner)
 {
-    const size_t cap = (size_t)spt * (size_t)block;
-    size_t b = (size_t)L * 4 + cap * 4 /*ids*/ + cap * 4 /*seg start*/ + (cap + 1) * 4 /*seg prefix*/ +
-               ((size_t)block / 64 + 1) * 4;
-    if (det) b += (size_t)L * 8 /*cursor, start*/ + (size_t)ept * (size_t)block * 6 /*w, key*/;
-    return b;
+    const size_t cap = (size_t)spt * (size_t)block, wd = (size_t)ept * (size_t)block;
+    size_t b = (size_t)acc_floats(L) * 4 + cap * 8 + scan_bytes(block);
+    if (!owner) b += (cap * 4 + 15) & ~(size_t)15;
+    if (det) return b + wd * 6 + (size_t)L * 8;
+    return b + (owner ? wd * 4 : 0);
 }
 
-// Entry f of the current pass (concatenated segments): binary search for
-// the last spike a with segp[a] <= f, then the table entry.
 // One {target, weight bits} table entry (generic or global pointer).
 __device__ __forceinline__ uint2 entry_at(const uint2 *p, size_t i) { return p[i]; }
 __device__ __forceinline__ uint2 entry_at(__attribute__((address_space(1))) const unsigned long long *p, size_t i)
@@ -477,51 +489,105 @@ __device__ __forceinline__ uint2 entry_at(__attribute__((address_space(1))) cons
     return make_uint2((uint32_t)x, (uint32_t)(x >> 32));
 }
 
-// Index of the last spike a with segp[a] <= f (segp ascending, segp[0] = 0).
-__device__ __forceinline__ int segment_of(const int *segp, int m, int f)
+// Zero this thread's kEPT positions of an owner map.
+template <int kEPT>
+__device__ __forceinline__ void clear_owner(uint16_t *own)
 {
-    int a = 0, b = m - 1;
-    while (a < b) {
-        const int mid = (a + b + 1) >> 1;
-        if (segp[mid] <= f) a = mid;
-        else b = mid - 1;
+    uint32_t *z = (uint32_t *)(own + (size_t)threadIdx.x * kEPT);
+#pragma unroll
+    for (int q = 0; q < kEPT / 2; ++q) z[q] = 0u;
+}
+
+// Owner map of a window: on entry own[] holds, at the window offset of each
+// non-empty segment's first entry, that segment's spike index, and zero
+// elsewhere (non-empty segments start at distinct offsets, and spike
+// indices grow with the offset).  A max-scan over the window, started from
+// *carry (the owner of the previous window's last entry), leaves in own[e]
+// the spike whose segment holds entry f0+e, and sets *carry for the next
+// window.  Thread i scans positions [i*kEPT, (i+1)*kEPT).  Begins with the
+// owner-map reads and ends with a workgroup barrier.
+template <int kEPT>
+__device__ __forceinline__ void owner_scan(uint16_t *own, int *s_wave, uint32_t *carry)
+{
+    const int lane = lane_id();
+    const int wave = (int)threadIdx.x / kWave;
+    const int nw = (int)blockDim.x / kWave;
+    uint32_t *ow = (uint32_t *)(own + (size_t)threadIdx.x * kEPT);
+    uint32_t raw[kEPT / 2];
+    uint32_t run = 0;
+#pragma unroll
+    for (int q = 0; q < kEPT / 2; ++q) {
+        raw[q] = ow[q];
+        run = max(run, max(raw[q] & 0xFFFFu, raw[q] >> 16));
     }
-    return a;
+    uint32_t incl = run;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)incl, off);
+        if (lane >= off) incl = max(incl, y);
+    }
+    uint32_t pre = (uint32_t)__shfl_up((int)incl, 1);
+    if (lane == 0) pre = 0;
+    if (lane == kWave - 1) s_wave[wave] = (int)incl;
+    __syncthreads();
+    pre = max(pre, *carry);
+    uint32_t tot = *carry;
+    for (int q = 0; q < nw; ++q) {  // uniform
+        const uint32_t y = (uint32_t)s_wave[q];
+        if (q < wave) pre = max(pre, y);
+        tot = max(tot, y);
+    }
+#pragma unroll
+    for (int q = 0; q < kEPT / 2; ++q) {
+        const uint32_t lo16 = max(pre, raw[q] & 0xFFFFu);
+        pre = max(lo16, raw[q] >> 16);
+        ow[q] = lo16 | (pre << 16);
+    }
+    *carry = tot;
+    __syncthreads();
 }
 
 // acc[0..len) <- input of slice p from the rows that spiked at step t-1.
 // Three dependent global levels, each with all its loads in flight:
 //  1. the step's bitmap, compacted in LDS into the ascending spike list
 //     (thread i owns a contiguous run of words);
-//  2. thread per spike: the row's segment bounds for slice p (offset table);
-//  3. thread per synapse entry of the concatenated segments (binary search
-//     over the LDS prefix of segment lengths).
-// Spikes beyond kSPT*B are handled in further passes.  Ends with
-// a workgroup barrier.
+//  2. thread per spike: the row's segment bounds for slice p (offset table),
+//     and a scan of the segment lengths gives each segment's offset in the
+//     concatenation of the pass's segments;
+//  3. thread per synapse entry of that concatenation, by windows of Wd
+//     entries: an owner map (segment heads, then a max-scan) or a binary
+//     search over the segment offsets names each entry's spike, and one
+//     table load per entry follows.
+// Spikes beyond kSPT*B are handled in further passes.  dyn is the LDS
+// workspace (acc first).  Ends with a workgroup barrier.
 // kDet = false: LDS float atomics, any order.
 // kDet = true: every acc[j] is summed in ascending presynaptic id from 0.0f,
 // bit-equal to a sequential push.  The entries of a pass are in ascending
 // (source, target) order; each window of them is counting-sorted by target
 // in LDS (count, scan, place) and each target then adds its entries in key
 // order, so the sum never depends on the order of the LDS atomics.
-template <bool kDet, int kEPT, int kSPT, class Net>
-__device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int lo, int len, float *acc,
-                                                 uint32_t *ws)
+template <bool kDet, int kEPT, int kSPT, bool kOwner, class Net>
+__device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int lo, int len, float *dyn)
 {
     const int B = (int)blockDim.x;
     const int tid = (int)threadIdx.x;
     const int cap = kSPT * B;
-    uint32_t *ids = ws;
-    uint32_t *seg0 = ws + cap;
-    int *segp = (int *)(ws + 2 * cap);
-    int *scratch = (int *)(ws + 3 * cap + 1);
-    // det workspace (after the scan scratch)
-    const int L = d.L;
-    int *dcur = scratch + B / kWave + 1;
-    int *dstart = dcur + L;
     const int Wd = kEPT * B;
-    float *dw = (float *)(dstart + L);
-    uint16_t *dkey = (uint16_t *)(dw + Wd);
+    const int L = d.L;
+    float *acc = dyn;
+    char *w8 = (char *)(dyn + acc_floats(L));
+    uint64_t *base = (uint64_t *)w8;
+    uint32_t *ids = (uint32_t *)w8;
+    int *scratch = (int *)(w8 + (size_t)cap * 8);
+    char *w1 = w8 + (size_t)cap * 8 + scan_bytes(B);
+    int *segp = (int *)w1;  // search lookup
+    char *w2 = kOwner ? w1 : w1 + (((size_t)cap * 4 + 15) & ~(size_t)15);
+    float *dw = (float *)w2;
+    uint16_t *own0 = (uint16_t *)(kDet ? w2 + (size_t)Wd * 4 : w2);
+    uint16_t *own1 = kDet ? own0 : own0 + Wd;
+    uint16_t *dkey = (uint16_t *)(w2 + (size_t)Wd * 4);  // det: also the owner map
+    int *dcur = (int *)(w2 + (size_t)Wd * 6);
+    int *dstart = dcur + L;
     for (int i = tid; i < len; i += B) {
         acc[i] = 0.0f;
         if (kDet) dcur[i] = 0;
@@ -545,26 +611,37 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
     }
     for (int w = wb + kKeep; w < we; ++w) mine += __popc(bm[w]);
     int total = 0;
-    const int base = block_exclusive_scan(mine, scratch, &total);
-    const size_t stride = (size_t)d.P + 1;
+    const int first = block_exclusive_scan(mine, scratch, &total);
+    const uint32_t stride = (uint32_t)d.P + 1u;
     // targets owned by this thread in the ordered phases: a contiguous run
     const int tper = (len + B - 1) / B;
     const int tb = min(tid * tper, len), te = min(tb + tper, len);
-    for (int pass = 0; pass < total; pass += cap) {
-        // 1. spike ids of this pass, ascending
-        if (mine > 0 && base < pass + cap && base + mine > pass) {
-            int idx = base;
+    // 1. spike ids of the first pass, ascending, from the kept words; later
+    // passes read the bitmap again
+    if (mine > 0 && first < cap) {
+        int idx = first;
 #pragma unroll
-            for (int k = 0; k < kKeep; ++k) {
-                uint32_t bits = kept[k];
-                while (bits) {
-                    if (idx >= pass && idx < pass + cap)
-                        ids[idx - pass] = (uint32_t)(wb + k) * 32u + (uint32_t)__builtin_ctz(bits);
-                    bits &= bits - 1u;
-                    ++idx;
-                }
+        for (int k = 0; k < kKeep; ++k) {
+            uint32_t bits = kept[k];
+            while (bits) {
+                if (idx < cap) ids[idx] = (uint32_t)(wb + k) * 32u + (uint32_t)__builtin_ctz(bits);
+                bits &= bits - 1u;
+                ++idx;
             }
-            for (int w = wb + kKeep; w < we && idx < pass + cap; ++w) {
+        }
+        for (int w = wb + kKeep; w < we && idx < cap; ++w) {
+            uint32_t bits = bm[w];
+            while (bits) {
+                if (idx < cap) ids[idx] = (uint32_t)w * 32u + (uint32_t)__builtin_ctz(bits);
+                bits &= bits - 1u;
+                ++idx;
+            }
+        }
+    }
+    for (int pass = 0; pass < total; pass += cap) {
+        if (pass > 0 && mine > 0 && first < pass + cap && first + mine > pass) {
+            int idx = first;
+            for (int w = wb; w < we && idx < pass + cap; ++w) {
                 uint32_t bits = bm[w];
                 while (bits) {
                     if (idx >= pass && idx < pass + cap) ids[idx - pass] = (uint32_t)w * 32u + (uint32_t)__builtin_ctz(bits);
@@ -576,45 +653,95 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
         __syncthreads();
         const int m = min(cap, total - pass);
         // 2. segment bounds, thread per spike
-        int lens[kSPT];
+        uint32_t sid[kSPT], e0[kSPT];
+        int lens[kSPT], head[kSPT];
         int lsum = 0;
 #pragma unroll
         for (int k = 0; k < kSPT; ++k) {
-            const int i = tid * kSPT + k;
+            sid[k] = e0[k] = 0;
             lens[k] = 0;
-            if (i < m) {
-                const auto o = d.soff + (size_t)ids[i] * stride + (size_t)p;
-                const uint32_t e0 = o[0], e1 = o[1];
-                seg0[i] = e0;
-                lens[k] = (int)(e1 - e0);
-            }
-            lsum += lens[k];
         }
+        if (tid * kSPT < m) {
+            // rows clamped to the pass's last spike: every load issues
+            // before the first wait
+            uint32_t e1[kSPT];
+#pragma unroll
+            for (int k = 0; k < kSPT; ++k) sid[k] = ids[min(tid * kSPT + k, m - 1)];
+#pragma unroll
+            for (int k = 0; k < kSPT; ++k) {
+                const auto o = d.soff + ((uint64_t)sid[k] * stride + (uint32_t)p);
+                e0[k] = o[0];
+                e1[k] = o[1];
+            }
+#pragma unroll
+            for (int k = 0; k < kSPT; ++k) lens[k] = tid * kSPT + k < m ? (int)(e1[k] - e0[k]) : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < kSPT; ++k) lsum += lens[k];
+        if (kOwner) clear_owner<kEPT>(own0);  // the first window's map, before the scan's barriers
         int E = 0;
         int run = block_exclusive_scan(lsum, scratch, &E);
+        // base[i] overwrites ids[2i..2i+1]: every id was read before the scan
 #pragma unroll
         for (int k = 0; k < kSPT; ++k) {
             const int i = tid * kSPT + k;
-            if (i < m) segp[i] = run;
+            // offset of the segment's first entry in the pass, -1 if empty
+            head[k] = lens[k] > 0 ? run : -1;
+            if (i < m) {
+                base[i] = (uint64_t)sid[k] * (uint64_t)d.nsyn + (uint64_t)e0[k] - (uint64_t)run;
+                if (!kOwner) segp[i] = run;
+            }
             run += lens[k];
         }
-        if (tid == 0) segp[m] = E;
-        __syncthreads();
-        // 3. entries, thread per synapse, windows of kEPT*B entries with all
-        // of a window's loads in flight.  Lanes take consecutive entries
+        if (!kOwner) __syncthreads();
+        // 3. entries, thread per synapse, windows of Wd entries with all of a
+        // window's loads in flight.  Lanes take consecutive entries
         // (f = f0 + k*B + tid): the loads coalesce, and a target's
         // contributions reach the LDS atomics nearly in ascending source
         // order, which keeps atomic mode on the ordered sum for many steps.
-        for (int f0 = 0; f0 < E; f0 += Wd) {
+        uint32_t carry = 0;
+        for (int f0 = 0, win = 0; f0 < E; f0 += Wd, ++win) {
+            int sa[kEPT];
+            if (kOwner) {
+                uint16_t *own = (win & 1) ? own1 : own0;
+#pragma unroll
+                for (int k = 0; k < kSPT; ++k)
+                    if (head[k] >= f0 && head[k] < f0 + Wd) own[head[k] - f0] = (uint16_t)(tid * kSPT + k);
+                __syncthreads();
+                // atomic mode: the other map is the next window's; its last
+                // readers (the previous window's entries) passed the barrier above
+                if (!kDet) clear_owner<kEPT>((win & 1) ? own0 : own1);
+                owner_scan<kEPT>(own, scratch, &carry);
+#pragma unroll
+                for (int k = 0; k < kEPT; ++k) sa[k] = own[k * B + tid];
+            } else {
+                // the last spike a < m with segp[a] <= f, which holds entry f
+                // (segp ascending from segp[0] = 0; empty segments repeat
+                // their successor's offset): branchless, kEPT searches in
+                // lockstep, log2(m) uniform steps.  A probe past m-1 reads
+                // m-1 instead, which is the answer whenever it passes.
+#pragma unroll
+                for (int k = 0; k < kEPT; ++k) sa[k] = 0;
+                for (int step = m > 1 ? 1 << (31 - __builtin_clz((unsigned)(m - 1))) : 0; step > 0; step >>= 1) {
+                    int c[kEPT], v[kEPT];
+#pragma unroll
+                    for (int k = 0; k < kEPT; ++k) {
+                        c[k] = min(sa[k] + step, m - 1);
+                        v[k] = segp[c[k]];
+                    }
+#pragma unroll
+                    for (int k = 0; k < kEPT; ++k) sa[k] = v[k] <= f0 + k * B + tid ? c[k] : sa[k];
+                }
+            }
+            // every lookup (sa < m) before the first load
+            uint64_t idx[kEPT];
+#pragma unroll
+            for (int k = 0; k < kEPT; ++k) idx[k] = base[sa[k]] + (uint64_t)(f0 + k * B + tid);
             uint2 x[kEPT];
 #pragma unroll
             for (int k = 0; k < kEPT; ++k) {
-                const int f = f0 + k * B + tid;
                 x[k] = make_uint2(0xFFFFFFFFu, 0u);
-                if (f < E) {
-                    const int sa = segment_of(segp, m, f);
-                    x[k] = entry_at(d.syn, (size_t)ids[sa] * (size_t)d.nsyn + (size_t)seg0[sa] + (size_t)(f - segp[sa]));
-                }
+                if (f0 + k * B + tid < E) x[k] = entry_at(d.syn, (size_t)idx[k]);
             }
             if (!kDet) {
 #pragma unroll
@@ -656,7 +783,8 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
                         dw[s0 + r] = u2f(x[k].y);
                     }
                 __syncthreads();
-                // each target adds its bucket in order
+                // each target adds its bucket in order; the keys are read,
+                // so the map is cleared for the next window
                 for (int j = tb; j < te; ++j) {
                     const int s0 = dstart[j], s1 = dcur[j];
                     float a = acc[j];
@@ -664,6 +792,7 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
                     acc[j] = a;
                     dcur[j] = 0;
                 }
+                if (kOwner) clear_owner<kEPT>(own0);
                 __syncthreads();
             }
         }
@@ -717,7 +846,7 @@ __global__ void __launch_bounds__(256) k_reduce_counts(DevNet d, int steps)
 template <int R, bool kDet>
 __global__ void __launch_bounds__(1024) k_slice_step(DevNet d, int rel)
 {
-    extern __shared__ float dyn[];
+    extern __shared__ __align__(16) float dyn[];  // slice_accumulate needs 16-byte alignment
     __shared__ int s_cnt;
     float *acc = dyn;
     const int t = d.tinfo[0] + rel;
@@ -742,7 +871,7 @@ __global__ void __launch_bounds__(1024) k_slice_step(DevNet d, int rel)
             I0[k] = ((j < d.n_exc) ? d.ge : d.gi) * thalamic(d.seed, (uint32_t)t, (uint32_t)j);
         }
     }
-    slice_accumulate<kDet, kEntriesSingle, kSpikesSingle>(d, t, p, lo, len, acc, (uint32_t *)(dyn + d.L));
+    slice_accumulate<kDet, kEntriesSingle, kSpikesSingle, kOwnerSingle>(d, t, p, lo, len, dyn);
     uint64_t masks[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
@@ -812,7 +941,7 @@ __device__ __forceinline__ void batch_unit(const BNet &d, int rel, int p, float 
     float *acc = dyn;
     const int B = (int)blockDim.x;
     const int tid = (int)threadIdx.x;
-    slice_accumulate<kDet, kEntriesBatch, spikes_batch<kDet>()>(d, t, p, lo, len, acc, (uint32_t *)(dyn + d.L));
+    slice_accumulate<kDet, kEntriesBatch, spikes_batch<kDet>(), kOwnerBatch>(d, t, p, lo, len, dyn);
     const bool last = t == d.last_t;
     if (tid == 0) *s_cnt = 0;
     __syncthreads();
@@ -836,7 +965,7 @@ template <bool kDet, int kB>
 __global__ void __launch_bounds__(kB, kB == 256 ? (kDet ? 6 : 8) : 1) k_batch_step(const DevNet *__restrict__ descs,
                                                                      const uint2 *__restrict__ work, int rel)
 {
-    extern __shared__ float dyn[];
+    extern __shared__ __align__(16) float dyn[];  // slice_accumulate needs 16-byte alignment
     __shared__ int s_cnt;
     const uint2 wu = work[blockIdx.x];
     if (wu.x == kNoWork) return;
@@ -862,7 +991,7 @@ template <bool kDet>
 __global__ void __launch_bounds__(256, 6) k_batch_chunk(const DevNet *__restrict__ descs, unsigned long long *units,
                                                      int32_t *ctr, int32_t *done, int total)
 {
-    extern __shared__ float dyn[];
+    extern __shared__ __align__(16) float dyn[];  // slice_accumulate needs 16-byte alignment
     __shared__ int s_cnt;
     __shared__ int s_pos, s_pub, s_base;
     __shared__ unsigned long long s_unit;
@@ -939,10 +1068,9 @@ __global__ void __launch_bounds__(256) k_batch_counts(const DevNet *descs, const
 template <int R, bool kDet>
 __global__ void __launch_bounds__(1024) k_skl_slice(DevNet d, int t0, int steps, int last_t)
 {
-    extern __shared__ float dyn[];
+    extern __shared__ __align__(16) float dyn[];  // slice_accumulate needs 16-byte alignment
     __shared__ int s_cnt;
     float *acc = dyn;
-    uint32_t *ws = (uint32_t *)(dyn + d.L);
     const int p = slice_of_block((int)blockIdx.x, (int)gridDim.x);
     const int lo = p * d.L;
     const int len = min(d.L, d.n - lo);
@@ -966,7 +1094,7 @@ __global__ void __launch_bounds__(1024) k_skl_slice(DevNet d, int t0, int steps,
     uint32_t epoch = 0;
     for (int rel = 0; rel < steps; ++rel) {
         const int t = t0 + rel;
-        slice_accumulate<kDet, kEntriesSingle, kSpikesSingle>(d, t, p, lo, len, acc, ws);
+        slice_accumulate<kDet, kEntriesSingle, kSpikesSingle, kOwnerSingle>(d, t, p, lo, len, dyn);
         uint64_t masks[R];
 #pragma unroll
         for (int k = 0; k < R; ++k) {
@@ -1275,13 +1403,13 @@ hipError_t launch_skl_slice_r(const DevNet &d, int t0, int steps, int last_t, co
 {
     DevNet dd = d;
     void *args[] = {&dd, &t0, &steps, &last_t};
-    return coop((const void *)k_skl_slice<R, kDet>, L, args, slice_lds_bytes(d.L, L.block, kDet, kEntriesSingle, kSpikesSingle));
+    return coop((const void *)k_skl_slice<R, kDet>, L, args, slice_lds_bytes(d.L, L.block, kDet, kEntriesSingle, kSpikesSingle, kOwnerSingle));
 }
 
 template <int R, bool kDet>
 hipError_t launch_slice_step_r(const DevNet &d, int rel, const Launch &L)
 {
-    return plain_launch(k_slice_step<R, kDet>, L, (uint32_t)slice_lds_bytes(d.L, L.block, kDet, kEntriesSingle, kSpikesSingle), d,
+    return plain_launch(k_slice_step<R, kDet>, L, (uint32_t)slice_lds_bytes(d.L, L.block, kDet, kEntriesSingle, kSpikesSingle, kOwnerSingle), d,
                         rel);
 }
 
@@ -1393,16 +1521,16 @@ int skl_slice_blocks_per_cu(int L, int block, bool det)
     int per_cu = 0;
     const int r = (L + block - 1) / block;
     const void *fn = det ? skl_slice_fn<true>(r) : skl_slice_fn<false>(r);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, slice_lds_bytes(L, block, det, kEntriesSingle, kSpikesSingle)) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, slice_lds_bytes(L, block, det, kEntriesSingle, kSpikesSingle, kOwnerSingle)) !=
         hipSuccess)
         return 0;
     return per_cu;
 }
 
-size_t slice_lds(int L, int block, bool det) { return slice_lds_bytes(L, block, det, kEntriesSingle, kSpikesSingle); }
+size_t slice_lds(int L, int block, bool det) { return slice_lds_bytes(L, block, det, kEntriesSingle, kSpikesSingle, kOwnerSingle); }
 size_t batch_lds(int L, bool det, int block)
 {
-    return slice_lds_bytes(L, block, det, kEntriesBatch, det ? spikes_batch<true>() : spikes_batch<false>());
+    return slice_lds_bytes(L, block, det, kEntriesBatch, det ? spikes_batch<true>() : spikes_batch<false>(), kOwnerBatch);
 }
 
 hipError_t launch_pull_thread(const DevNet &d, int t, int rel, int last, const Launch &L)
