@@ -560,7 +560,7 @@ __device__ __forceinline__ void owner_scan(uint16_t *own, int *s_wave, uint32_t 
 //     table load per entry follows.
 // Spikes beyond kSPT*B are handled in further passes.  dyn is the LDS
 // workspace (acc first).  Ends with a workgroup barrier.
-// kDet = false: LDS float atomics, any order.
+// kDet = false: binary32 adds applied atomically (compare-and-swap), any order.
 // kDet = true: every acc[j] is summed in ascending presynaptic id from 0.0f,
 // bit-equal to a sequential push.  The entries of a pass are in ascending
 // (source, target) order; each window of them is counting-sorted by target
@@ -744,9 +744,32 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
                 if (f0 + k * B + tid < E) x[k] = entry_at(d.syn, (size_t)idx[k]);
             }
             if (!kDet) {
+                // binary32 adds applied atomically, in any order, as
+                // compare-and-swap loops on the bit patterns: every first
+                // attempt of the thread is issued before any result is
+                // looked at, and a lost race retries from the value it saw.
+                // On MI355X this runs the batched step 1.5x faster than the
+                // LDS float atomic add (ds_add_f32).
+                uint32_t *ai = (uint32_t *)acc;
+                uint32_t seen[kEPT], got[kEPT];
 #pragma unroll
-                for (int k = 0; k < kEPT; ++k)
-                    if (x[k].x != 0xFFFFFFFFu) atomicAdd(&acc[(int)x[k].x - lo], u2f(x[k].y));
+                for (int k = 0; k < kEPT; ++k) seen[k] = x[k].x != 0xFFFFFFFFu ? ai[(int)x[k].x - lo] : 0u;
+#pragma unroll
+                for (int k = 0; k < kEPT; ++k) {
+                    got[k] = seen[k];
+                    if (x[k].x != 0xFFFFFFFFu)
+                        got[k] = atomicCAS(&ai[(int)x[k].x - lo], seen[k],
+                                           __float_as_uint(__uint_as_float(seen[k]) + u2f(x[k].y)));
+                }
+#pragma unroll
+                for (int k = 0; k < kEPT; ++k) {
+                    uint32_t cur = got[k];
+                    while (cur != seen[k]) {  // lost a race: at least one other add landed
+                        seen[k] = cur;
+                        cur = atomicCAS(&ai[(int)x[k].x - lo], seen[k],
+                                        __float_as_uint(__uint_as_float(seen[k]) + u2f(x[k].y)));
+                    }
+                }
             } else {
                 // ordered: counting sort of the window by target, then each
                 // target adds its entries in window-offset (= source) order
@@ -961,8 +984,10 @@ __device__ __forceinline__ void batch_unit(const BNet &d, int rel, int p, float 
 // One launch per step: workgroup b takes work unit work[b] = {slot, p} at
 // step rel; the host orders the units so that each XCD holds whole
 // instances.  Instances whose chunk ends before rel do nothing.
+// 256-thread workgroups: 6 (det) or 7 (atomic) waves per SIMD, the most
+// that keep every value in registers.
 template <bool kDet, int kB>
-__global__ void __launch_bounds__(kB, kB == 256 ? (kDet ? 6 : 8) : 1) k_batch_step(const DevNet *__restrict__ descs,
+__global__ void __launch_bounds__(kB, kB == 256 ? (kDet ? 6 : 7) : 1) k_batch_step(const DevNet *__restrict__ descs,
                                                                      const uint2 *__restrict__ work, int rel)
 {
     extern __shared__ __align__(16) float dyn[];  // slice_accumulate needs 16-byte alignment
