@@ -245,13 +245,14 @@ def bench_serve(g, args):
 
 
 def extras(g, args):
-    """Batched and served summaries beside the single-network line: 256
-    networks in one batch (det mode, 1000 steps), then open-loop serving at
-    50 req/s for 20 s and an overload run at 400 req/s for 10 s."""
+    """Batched and served summaries beside the single-network line, in the
+    same accumulation mode: 256 networks in one batch (1000 steps), then
+    open-loop serving at 50 req/s for 20 s and an overload run at 400 req/s
+    for 10 s."""
     import copy
     t0 = time.perf_counter()
     a = copy.copy(args)
-    a.batch, a.accum, a.steps, a.warmup, a.plain_sample, a.slice_len, a.tune = 256, "det", 1000, 100, 8, 0, False
+    a.batch, a.steps, a.warmup, a.plain_sample, a.slice_len, a.tune = 256, 1000, 100, 8, 0, False
     b = bench_batch(g, a)
     print(f"batched done in {time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
     s1 = copy.copy(a)
@@ -262,11 +263,11 @@ def extras(g, args):
     sat = bench_serve(g, s2)
     print(f"served done in {time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
     return {
-        "batched": {"workload": b["config"]["workload"], "accum": "det", "sim_s_per_s": b["value"],
+        "batched": {"workload": b["config"]["workload"], "accum": a.accum, "sim_s_per_s": b["value"],
                     "events_per_s": b["events_per_s"], "ms_per_step": b["ms_per_step"],
                     "plain_sim_s_per_s": b["plain"]["sim_s_per_s"], "speedup_vs_plain": b["speedup_vs_plain"],
                     "roofline": b["roofline"]},
-        "served": {"workload": sv["config"]["workload"], "accum": "det", "p50_s": sv["latency_s"]["p50"],
+        "served": {"workload": sv["config"]["workload"], "accum": a.accum, "p50_s": sv["latency_s"]["p50"],
                    "p99_s": sv["latency_s"]["p99"], "sim_s_per_s": sv["value"],
                    "offered_sim_s_per_s": sv["config"]["offered_sim_s_per_s"],
                    "overload": {"rate_req_s": 400.0, "sim_s_per_s": sat["value"], "p50_s": sat["latency_s"]["p50"],

@@ -493,9 +493,14 @@ __device__ __forceinline__ uint2 entry_at(__attribute__((address_space(1))) cons
 template <int kEPT>
 __device__ __forceinline__ void clear_owner(uint16_t *own)
 {
+    // The zero passes through an empty asm operand so that it is made here:
+    // hoisted out of the window loop, the zero vector of the 16-byte store
+    // was kept in scratch memory.
+    uint32_t zero = 0u;
+    asm volatile("" : "+v"(zero));
     uint32_t *z = (uint32_t *)(own + (size_t)threadIdx.x * kEPT);
 #pragma unroll
-    for (int q = 0; q < kEPT / 2; ++q) z[q] = 0u;
+    for (int q = 0; q < kEPT / 2; ++q) z[q] = zero;
 }
 
 // Owner map of a window: on entry own[] holds, at the window offset of each
@@ -656,12 +661,13 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
         uint32_t sid[kSPT], e0[kSPT];
         int lens[kSPT], head[kSPT];
         int lsum = 0;
+        if (tid * kSPT >= m) {
 #pragma unroll
-        for (int k = 0; k < kSPT; ++k) {
-            sid[k] = e0[k] = 0;
-            lens[k] = 0;
-        }
-        if (tid * kSPT < m) {
+            for (int k = 0; k < kSPT; ++k) {
+                sid[k] = e0[k] = 0;
+                lens[k] = 0;
+            }
+        } else {
             // rows clamped to the pass's last spike: every load issues
             // before the first wait
             uint32_t e1[kSPT];
@@ -984,10 +990,10 @@ __device__ __forceinline__ void batch_unit(const BNet &d, int rel, int p, float 
 // One launch per step: workgroup b takes work unit work[b] = {slot, p} at
 // step rel; the host orders the units so that each XCD holds whole
 // instances.  Instances whose chunk ends before rel do nothing.
-// 256-thread workgroups: 6 (det) or 7 (atomic) waves per SIMD, the most
-// that keep every value in registers.
+// 256-thread workgroups: 7 waves per SIMD, the most that keep every value
+// in registers (and whose LDS fits a CU at 512-neuron slices).
 template <bool kDet, int kB>
-__global__ void __launch_bounds__(kB, kB == 256 ? (kDet ? 6 : 7) : 1) k_batch_step(const DevNet *__restrict__ descs,
+__global__ void __launch_bounds__(kB, kB == 256 ? 7 : 1) k_batch_step(const DevNet *__restrict__ descs,
                                                                      const uint2 *__restrict__ work, int rel)
 {
     extern __shared__ __align__(16) float dyn[];  // slice_accumulate needs 16-byte alignment
