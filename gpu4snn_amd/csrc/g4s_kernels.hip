@@ -416,7 +416,10 @@ __device__ __forceinline__ void slice_count(const DevNet &d, int rel, int p, int
 }
 
 // Exclusive prefix sum over the workgroup of one value per thread; returns
-// the thread's prefix and sets *total.  s_tmp holds blockDim/64 + 1 ints.
+// the thread's prefix and sets *total.  s_tmp holds blockDim/64 ints.  One
+// barrier: every thread reads all wave totals.  The caller guarantees a
+// workgroup barrier between the reads of the previous use of s_tmp and
+// this call.
 __device__ __forceinline__ int block_exclusive_scan(int v, int *s_tmp, int *total)
 {
     const int lane = lane_id();
@@ -428,21 +431,16 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int *s_tmp, int *tota
         const int y = __shfl_up(incl, off);
         if (lane >= off) incl += y;
     }
-    __syncthreads();
     if (lane == kWave - 1) s_tmp[wave] = incl;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        int run = 0;
-        for (int w = 0; w < nw; ++w) {
-            const int c = s_tmp[w];
-            s_tmp[w] = run;
-            run += c;
-        }
-        s_tmp[nw] = run;
+    int pre = 0, tot = 0;
+    for (int w = 0; w < nw; ++w) {  // uniform
+        const int c = s_tmp[w];
+        pre += w < wave ? c : 0;
+        tot += c;
     }
-    __syncthreads();
-    *total = s_tmp[nw];
-    return s_tmp[wave] + incl - v;
+    *total = tot;
+    return pre + incl - v;
 }
 
 // Spikes per pass per thread and entry loads in flight per thread:
