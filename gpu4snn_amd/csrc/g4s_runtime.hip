@@ -261,7 +261,7 @@ int coop_launch(g4s_net *net, CoopKernel k, int block, int64_t useful_threads, L
     return G4S_OK;
 }
 
-// Slice partition for the LDS-accumulating schemes: `want` slices (0 = two
+// Slice partition for the LDS-accumulating schemes: `want` slices (0 = one
 // per CU), slice length L a multiple of 64 and at most 16*1024 neurons;
 // offsets stored as uint16 (Nsyn < 65536).  Rebuilt when the slice count
 // changes.
@@ -277,7 +277,7 @@ int build_slices(g4s_net *net, int want)
     if (net->d.nsyn > 65535) return fail(G4S_ERR_UNSUPPORTED, "slice schemes need Nsyn < 65536");
     if (net->cus == 0) net->cus = device_cus();
     const int64_t n = net->d.n;
-    if (want <= 0) want = 2 * std::max(net->cus, 1);
+    if (want <= 0) want = std::max(net->cus, 1);
     const int64_t L = pick_slice_len(n, want);
     const int64_t P = (n + L - 1) / L;
     if (net->slices_ready && net->d.L == L) return G4S_OK;
@@ -309,6 +309,10 @@ int build_slices(g4s_net *net, int want)
 
 // Workgroup size of the slice kernels: requested, or one neuron per thread
 // up to 1024 threads.
+// Slice kernels default to one thread per neuron of the slice (at most 1024).
+// More threads per slice run faster (the autotuner picks 1024 at N=1e5), but
+// in atomic mode they let the adds of a target arrive further from source
+// order, and the first steps then drift from the ordered sum sooner.
 int slice_block(const g4s_net *net, int requested)
 {
     if (requested > 0) return requested;
@@ -532,8 +536,20 @@ int run_impl(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, 
     const bool slice = alg == G4S_ALG_AB || alg == G4S_ALG_SKL;
     const bool det = accum == G4S_ACCUM_DET;
     if (det && !slice && (rc = build_det_tables(net))) return rc;
-    if (slice && (rc = build_slices(net, o.slices))) return rc;
-    const int sblock = slice ? slice_block(net, o.block) : 0;
+    int sblock = 0;
+    if (slice && o.slices > 0) {
+        if ((rc = build_slices(net, o.slices))) return rc;
+        sblock = slice_block(net, o.block);
+    } else if (slice) {
+        // default tiling: about one slice per CU; more slices while a
+        // slice's workspace would not fit the LDS of one workgroup
+        if (net->cus == 0) net->cus = device_cus();
+        for (int64_t want = std::max(net->cus, 1);; want *= 2) {
+            if ((rc = build_slices(net, (int)want))) return rc;
+            sblock = slice_block(net, o.block);
+            if (slice_lds(net->d.L, sblock, det) <= 64 * 1024 || net->d.L <= 64) break;
+        }
+    }
     if (slice && (int64_t)net->d.L > 16 * (int64_t)sblock)
         return fail(G4S_ERR_ARG, "block %d too small for slices of %d neurons (max 16 per thread)", sblock, net->d.L);
     if (slice && slice_lds(net->d.L, sblock, det) > 64 * 1024)

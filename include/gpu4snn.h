@@ -57,8 +57,10 @@ typedef struct g4s_net g4s_net;
 /* Tuning of one run; zero-initialised fields select the defaults. */
 typedef struct g4s_run_opts {
     int32_t block;       /* threads per workgroup (multiple of 64; <= 256, or <= 1024
-                            for AB/SKL), 0 = chosen per scheme                     */
-    int32_t slices;      /* AB/SKL: target slices P (workgroups), 0 = auto          */
+                            for AB/SKL), 0 = chosen per scheme (AB/SKL: one thread
+                            per neuron of a slice, at most 1024)                   */
+    int32_t slices;      /* AB/SKL: target slices P (workgroups), 0 = about one per
+                            CU, more if a slice's LDS workspace would not fit      */
     int32_t chunk;       /* steps per device chunk between host copies, 0 = 1000       */
     int32_t graph;       /* AB: >= 0 replays the chunk's per-step launches from a
                             captured HIP graph, < 0 launches them directly          */
