@@ -184,3 +184,25 @@ def test_autotune_plan_runs_and_is_cached(g4s, oracle):
         net.reset()
         _, c = net.run_plan(pa, 300, raster=False)
         assert c.sum() > 0
+
+
+def test_default_tiling_fits_det_workspace(g4s, oracle):
+    """Default AB tiling at N=400k: one slice per CU would need more than
+    64 KB of det workspace per workgroup, so the runtime doubles the slice
+    count until it fits; the run stays bit-identical to the oracle, and
+    atomic mode keeps one slice per CU."""
+    n, nsyn, seed, steps = 400_000, 100, 5, 20
+    onet = oracle.Network(n, nsyn, seed)
+    oraster, ocounts, _ = onet.run(steps)
+    with g4s.Network(n, nsyn, seed) as net:
+        words, counts = net.run(steps, alg="AB", accum="det")
+        p_det = net.timing()["grid"]
+        assert np.array_equal(counts, ocounts)
+        assert np.array_equal(g4s.raster_to_dense(words, n), oraster)
+        v, u, _, _ = net.state()
+        assert np.array_equal(v.view(np.uint32), onet.v.view(np.uint32))
+        net.reset()
+        net.run(steps, alg="AB", accum="atomic")
+        p_atomic = net.timing()["grid"]
+    # lengths round up to multiples of 64: 481 slices of 832 vs 250 of 1600
+    assert p_det > 3 * p_atomic // 2 > 0, (p_det, p_atomic)
