@@ -1295,7 +1295,6 @@ __global__ void __launch_bounds__(64) k_build_rows(uint64_t seed, int64_t n, int
     __shared__ uint8_t first[kDevRowMax];
     __shared__ uint16_t hi_list[kDevRowMax];
     __shared__ uint32_t taken[kDevRowMax / 32];
-    __shared__ int s_nhi;
     const int lane = (int)threadIdx.x;
     const uint64_t m = (uint64_t)(n - 1);
     const uint64_t J0 = m - (uint64_t)nsyn;
