@@ -120,10 +120,16 @@ def big():
     return dict(n=100_000, nsyn=1000, seed=20221)
 
 
-def test_cfg1_det_all_schemes_identical_and_oracle(g4s, oracle, big):
+@pytest.fixture(scope="module")
+def big_oracle(oracle, big):
+    """The oracle's first 60 steps of configuration 1."""
+    return oracle_raster(oracle, big["n"], big["nsyn"], 60, big["seed"])
+
+
+def test_cfg1_det_all_schemes_identical_and_oracle(g4s, big, big_oracle):
     n, nsyn, seed = big["n"], big["nsyn"], big["seed"]
     steps = 60
-    onet, oraster, _ = oracle_raster(oracle, n, nsyn, steps, seed)
+    onet, oraster, _ = big_oracle
     with g4s.Network(n, nsyn, seed) as net:
         post, w = net.synapses()
         assert np.array_equal(post, onet.post) and np.array_equal(w.view(np.uint32), onet.w.view(np.uint32))
@@ -138,6 +144,21 @@ def test_cfg1_det_all_schemes_identical_and_oracle(g4s, oracle, big):
             assert np.array_equal(words, ref), alg
         v, u, _, _ = net.state()
         assert np.array_equal(v.view(np.uint32), onet.v.view(np.uint32))
+
+
+@pytest.mark.parametrize("alg", ["AB", "SKL"])
+def test_cfg1_bench_plan_det_bitwise(g4s, big, big_oracle, alg):
+    """The tiling the bench times (256 slices requested, 1024 threads per
+    workgroup) is bit-identical to the oracle in det mode."""
+    n, nsyn, seed = big["n"], big["nsyn"], big["seed"]
+    onet, oraster, _ = big_oracle
+    with g4s.Network(n, nsyn, seed) as net:
+        words, _ = net.run(60, alg=alg, accum="det", slices=256, block=1024)
+        assert net.timing()["grid"] == 224
+        assert np.array_equal(dense(g4s, words, n), oraster)
+        v, u, _, _ = net.state()
+        assert np.array_equal(v.view(np.uint32), onet.v.view(np.uint32))
+        assert np.array_equal(u.view(np.uint32), onet.u.view(np.uint32))
 
 
 def test_cfg1_atomic_rates(g4s, big):
