@@ -135,9 +135,9 @@ def test_batch_equals_single_network(g4s):
 @pytest.mark.parametrize("schedule", ["queue", "step"])
 def test_batch_atomic_first_steps_and_rates(g4s, oracle, schedule):
     """First 100 steps identical per instance; the spike total of the three
-    instances over 4000 steps (~230k spikes, so decorrelated runs differ by
-    ~0.3% rms) within 1% of the oracle's."""
-    specs = [(2500, 1000, 51, 1.0, 4000), (2500, 1000, 52, 1.0, 4000), (4000, 500, 53, 1.2, 4000)]
+    instances over 8000 steps (~460k spikes, so decorrelated runs differ by
+    ~0.2% rms and the 1% bar is over 4 sigma) within 1% of the oracle's."""
+    specs = [(2500, 1000, 51, 1.0, 8000), (2500, 1000, 52, 1.0, 8000), (4000, 500, 53, 1.2, 8000)]
     with g4s.Batch("atomic", chunk=500, schedule=schedule) as b:
         slots = [b.add(n, ns, sd, st, thal_scale=sc) for (n, ns, sd, sc, st) in specs]
         b.run()

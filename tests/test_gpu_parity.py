@@ -29,9 +29,9 @@ def cfg0(oracle):
 # Firing rates of atomic mode are compared over RATE_STEPS steps: once float
 # atomics reorder an addition the trajectory decorrelates from the oracle's,
 # and two independent 1000-step runs at N=2500 (~18k spikes) differ by about
-# 1% (1 sigma) on their own.  At 10000 steps (~180k spikes) the spread of
-# the difference is ~0.35%, so the 1% bar is about 3 sigma.
-RATE_STEPS = 10_000
+# 1% (1 sigma) on their own.  At 20000 steps (~360k spikes) the spread of
+# the difference is ~0.25%, so the 1% bar is about 4 sigma.
+RATE_STEPS = 20_000
 
 
 @pytest.fixture(scope="module")
