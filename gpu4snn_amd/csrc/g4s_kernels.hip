@@ -364,12 +364,12 @@ __global__ void __launch_bounds__(256) k_ap(DevNet d, int t, int rel, int last, 
 }
 
 // ---------------------------------------------------------------------------
-// Slice-owned propagation (AB and SKL in atomic mode).  Workgroup p owns the
-// neurons of slice p.  Its input for step t is the part of every row queued
-// at step t-1 that targets slice p: a contiguous segment of the sorted row,
-// located by the offset table.  Segments are accumulated in LDS, so there
-// are no global float atomics and no pending-input array in HBM; the owner
-// then updates its neurons directly.
+// Slice-owned propagation (AB, SKL and the batched form, both modes).
+// Workgroup p owns the neurons of slice p.  Its input for step t is the part
+// of every row that spiked at step t-1 that targets slice p: a contiguous
+// segment of the sorted row, located by the offset table.  Segments are
+// accumulated in LDS, so there are no global float atomics and no
+// pending-input array in HBM; the owner then updates its neurons directly.
 // ---------------------------------------------------------------------------
 
 // Workgroups are dealt to the 8 XCDs round-robin; consecutive slices go to
