@@ -322,8 +322,12 @@ __global__ void __launch_bounds__(256) k_ap(DevNet d, int t, int rel, int last, 
     }
     __syncthreads();
     int32_t *ctl = d.ctl + rel * kCtlStride;
+    // Everything consumers read from this launch is published with agent-scope
+    // atomics (the count and the queue slots), and every count increment of
+    // this workgroup has returned before the barrier above: a relaxed
+    // increment suffices, and no release (an L2 write-back of the XCD) is paid.
     if (threadIdx.x == 0)
-        __hip_atomic_fetch_add(&ctl[kCtlDone], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(&ctl[kCtlDone], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 
     uint32_t *q = (t & 1) ? d.q1 : d.q0;
     float *out = ((t + 1) & 1) ? d.isyn1 : d.isyn0;
@@ -337,8 +341,8 @@ __global__ void __launch_bounds__(256) k_ap(DevNet d, int t, int rel, int last, 
             for (;;) {
                 if (item < ld_agent(&ctl[kCtlCount])) { go = 1; break; }
                 if (ld_agent(&ctl[kCtlDone]) == (int)gridDim.x) {
-                    // every producer has finished: the count is final
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                    // every producer has finished: the count is final (read
+                    // with an agent-scope load, no acquire fence needed)
                     go = item < ld_agent(&ctl[kCtlCount]);
                     break;
                 }
