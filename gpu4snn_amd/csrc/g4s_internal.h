@@ -23,7 +23,7 @@ int build_synapses(const g4s_spec *spec, uint2 *syn, hipStream_t stream);
 
 constexpr uint32_t kSentinel = 0xFFFFFFFFu;   // empty spike-queue slot
 constexpr int kCtlStride = 4;                 // per-step control words
-enum { kCtlCount = 0, kCtlHead = 1, kCtlDone = 2 };
+enum { kCtlCount = 0 };  // spikes of the step; k_ap keeps finished workgroups in word 1
 constexpr uint32_t kSpinLimit = 1u << 24;     // bounded device waits (~10 s)
 
 // Everything a kernel needs about one network, passed by value.

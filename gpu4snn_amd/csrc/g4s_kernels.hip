@@ -69,9 +69,6 @@ __device__ __forceinline__ bool neuron_step(const Net &d, int t, int j, float is
 // row of the chunk, and the queue entries appended at one atomic per wave.
 // Waves starting at or beyond `limit` (the end of the caller's neuron range)
 // write nothing: their words belong to another workgroup.
-// kPublish: queue slots are stored with agent-scope atomics because
-// consumers of the same launch read them (AP scheme).
-template <bool kPublish>
 __device__ __forceinline__ void emit(const DevNet &d, int t, int rel, int wave_base, bool spk, int j,
                                      int limit)
 {
@@ -91,11 +88,7 @@ __device__ __forceinline__ void emit(const DevNet &d, int t, int rel, int wave_b
         base = __shfl(base, 0);
         if (spk) {
             uint32_t *q = (t & 1) ? d.q1 : d.q0;
-            const int pos = base + bits_below(mask);
-            if (kPublish)
-                __hip_atomic_store(&q[pos], (uint32_t)j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            else
-                q[pos] = (uint32_t)j;
+            q[base + bits_below(mask)] = (uint32_t)j;
         }
     }
 }
@@ -253,7 +246,7 @@ __global__ void __launch_bounds__(256) k_update(DevNet d, int t, int rel, int la
         const int j = base + (int)threadIdx.x;
         bool spk = false;
         if (j < d.n) spk = neuron_step(d, t, j, take_input(d, t, j), last != 0);
-        emit<false>(d, t, rel, base + ((int)threadIdx.x & ~(kWave - 1)), spk, j, d.n);
+        emit(d, t, rel, base + ((int)threadIdx.x & ~(kWave - 1)), spk, j, d.n);
     }
 }
 
@@ -298,14 +291,26 @@ __global__ void __launch_bounds__(256) k_push_flat(DevNet d, int t, int rel)
 }
 
 // AP scheme: one co-resident launch per step.  Every workgroup updates its
-// share of neurons and publishes spikes into the queue, then becomes a
-// consumer that claims queued rows one at a time (atomic head) and
-// scatters them.  A row is claimed as soon as its slot is published, so
-// propagation overlaps the update.  A slot holds kSentinel until published.
+// share of neurons and publishes their spikes into the queue, then its waves
+// become consumers that scatter queued rows.  A row is scattered as soon as
+// its slot is published, so propagation overlaps the update.
+//
+// Control: ctl[kCtlCount] (low half) and ctl[kCtlCount + 1] (high half) form
+// one 64-bit word: spikes queued so far, and workgroups finished.  Each
+// workgroup adds its tile's spike total (and, with its last tile, one
+// finished workgroup) in a single atomic.  Device-scope atomics on one
+// address are serialised (about 30 ns each on MI355X), so the launch makes
+// one per tile and no per-row claims: consumer wave w takes queue items w,
+// w + W, w + 2W, ... (W waves in the grid).  It waits until its item is below
+// the count, or until the word shows every workgroup finished, which makes
+// the count final.  A slot holds kSentinel until its id is stored (the count
+// is raised first).  Everything consumers read from the launch is written
+// with agent-scope atomics, so neither a release (an L2 write-back) nor an
+// acquire (an L2 invalidate) is needed.
 // Step t re-arms the next step's buffer q[(t+1)&1] (only its first
 // count(t-1) slots can be non-sentinel); nothing reads that buffer during
 // step t, and consumed slots keep their ids so the last step's queue stays
-// valid for whichever scheme runs next.
+// valid for whichever scheme runs next.  Workgroups of at most 256 threads.
 __global__ void __launch_bounds__(256) k_ap(DevNet d, int t, int rel, int last, int prev_count)
 {
     const int stride = gridDim.x * blockDim.x;
@@ -314,44 +319,70 @@ __global__ void __launch_bounds__(256) k_ap(DevNet d, int t, int rel, int last, 
         const int used = t == 0 ? 0 : (rel ? d.ctl[(rel - 1) * kCtlStride + kCtlCount] : prev_count);
         for (int i = blockIdx.x * blockDim.x + (int)threadIdx.x; i < used; i += stride) qn[i] = kSentinel;
     }
+    uint64_t *word = reinterpret_cast<uint64_t *>(d.ctl + rel * kCtlStride + kCtlCount);
+    uint32_t *q = (t & 1) ? d.q1 : d.q0;
+    __shared__ int s_wcnt[256 / kWave];
+    __shared__ int s_base;
+    const int lane = lane_id();
+    const int wave = (int)threadIdx.x / kWave;
+    const int nwave = (int)blockDim.x / kWave;
     for (int base = blockIdx.x * blockDim.x; base < d.n; base += stride) {
         const int j = base + (int)threadIdx.x;
         bool spk = false;
         if (j < d.n) spk = neuron_step(d, t, j, take_input(d, t, j), last != 0);
-        emit<true>(d, t, rel, base + ((int)threadIdx.x & ~(kWave - 1)), spk, j, d.n);
+        const int wave_base = base + wave * kWave;
+        const uint64_t mask = __ballot(spk);
+        if (wave_base < d.n) {
+            uint32_t *bm = (t & 1) ? d.bm1 : d.bm0;
+            const int w = (wave_base >> 5) + lane;
+            if (lane < 2 && w < d.words) {
+                const uint32_t bits = lane ? (uint32_t)(mask >> 32) : (uint32_t)mask;
+                bm[w] = bits;
+                if (d.raster) d.raster[(size_t)rel * (size_t)d.words + (size_t)w] = bits;
+            }
+        }
+        if (lane == 0) s_wcnt[wave] = __popcll(mask);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int tot = 0;
+            for (int k = 0; k < nwave; ++k) tot += s_wcnt[k];
+            const uint64_t inc = (uint64_t)tot + (base + stride >= d.n ? (1ull << 32) : 0ull);
+            s_base = (int)(uint32_t)__hip_atomic_fetch_add(word, inc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+        if (spk) {
+            int pos = s_base + bits_below(mask);
+            for (int k = 0; k < wave; ++k) pos += s_wcnt[k];
+            __hip_atomic_store(&q[pos], (uint32_t)j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();  // s_wcnt and s_base are rewritten by the next tile
     }
-    __syncthreads();
-    int32_t *ctl = d.ctl + rel * kCtlStride;
-    // Everything consumers read from this launch is published with agent-scope
-    // atomics (the count and the queue slots), and every count increment of
-    // this workgroup has returned before the barrier above: a relaxed
-    // increment suffices, and no release (an L2 write-back of the XCD) is paid.
-    if (threadIdx.x == 0)
-        __hip_atomic_fetch_add(&ctl[kCtlDone], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 
-    uint32_t *q = (t & 1) ? d.q1 : d.q0;
     float *out = ((t + 1) & 1) ? d.isyn1 : d.isyn0;
-    const int lane = lane_id();
-    for (;;) {
-        int item = 0, go = 0;
+    const int waves = (int)gridDim.x * nwave;
+    int final_count = -1;  // known once every workgroup has finished
+    for (int item = (int)blockIdx.x * nwave + wave;; item += waves) {
+        int go = 0;
         uint32_t s = 0;
         if (lane == 0) {
-            item = atomicAdd(&ctl[kCtlHead], 1);
-            uint32_t spins = 0;
-            for (;;) {
-                if (item < ld_agent(&ctl[kCtlCount])) { go = 1; break; }
-                if (ld_agent(&ctl[kCtlDone]) == (int)gridDim.x) {
-                    // every producer has finished: the count is final (read
-                    // with an agent-scope load, no acquire fence needed)
-                    go = item < ld_agent(&ctl[kCtlCount]);
-                    break;
+            if (final_count >= 0) {
+                go = item < final_count;
+            } else {
+                uint32_t spins = 0;
+                for (;;) {
+                    const uint64_t v = __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (item < (int)(uint32_t)v) { go = 1; break; }
+                    if ((int)(v >> 32) == (int)gridDim.x) {
+                        final_count = (int)(uint32_t)v;
+                        break;
+                    }
+                    if (error_set(d)) break;
+                    if (++spins > kSpinLimit) { raise_error(d); break; }
+                    __builtin_amdgcn_s_sleep(1);
                 }
-                if (error_set(d)) break;
-                if (++spins > kSpinLimit) { raise_error(d); break; }
-                __builtin_amdgcn_s_sleep(1);
             }
             if (go) {
-                spins = 0;
+                uint32_t spins = 0;
                 for (;;) {
                     s = __hip_atomic_load(&q[item], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     if (s != kSentinel) break;
@@ -1169,7 +1200,7 @@ __global__ void __launch_bounds__(256) k_pull_thread(DevNet d, int t, int rel, i
         const int j = base + (int)threadIdx.x;
         bool spk = false;
         if (j < d.n) spk = neuron_step(d, t, j, gather_input(d, t, j), last != 0);
-        emit<false>(d, t, rel, base + ((int)threadIdx.x & ~(kWave - 1)), spk, j, d.n);
+        emit(d, t, rel, base + ((int)threadIdx.x & ~(kWave - 1)), spk, j, d.n);
     }
 }
 
@@ -1211,26 +1242,24 @@ __global__ void __launch_bounds__(256) k_pull_wave(DevNet d, int t, int rel, int
         const int j = wb + lane;
         bool spk = false;
         if (j < d.n) spk = neuron_step(d, t, j, mine, last != 0);
-        emit<false>(d, t, rel, wb, spk, j, d.n);
+        emit(d, t, rel, wb, spk, j, d.n);
     }
 }
 
-// det mode, AP scheme: one co-resident launch per step; waves claim
-// 64-neuron tiles from an atomic work counter (dynamic load balance).
+// det mode, AP scheme: one co-resident launch per step; wave w of W takes
+// the 64-neuron tiles w, w + W, w + 2W, ...  (A shared atomic tile counter
+// measured slower: its device-scope atomics on one address serialise.)
 __global__ void __launch_bounds__(256) k_ap_det(DevNet d, int t, int rel, int last)
 {
-    int32_t *ctl = d.ctl + rel * kCtlStride;
     const int lane = lane_id();
     const int tiles = (d.n + kWave - 1) / kWave;
-    for (;;) {
-        int tile = 0;
-        if (lane == 0) tile = atomicAdd(&ctl[kCtlHead], 1);
-        tile = __shfl(tile, 0);
-        if (tile >= tiles) break;
+    const int nwave = (int)blockDim.x / kWave;
+    const int waves = (int)gridDim.x * nwave;
+    for (int tile = (int)blockIdx.x * nwave + (int)threadIdx.x / kWave; tile < tiles; tile += waves) {
         const int j = tile * kWave + lane;
         bool spk = false;
         if (j < d.n) spk = neuron_step(d, t, j, gather_input(d, t, j), last != 0);
-        emit<false>(d, t, rel, tile * kWave, spk, j, d.n);
+        emit(d, t, rel, tile * kWave, spk, j, d.n);
     }
 }
 
