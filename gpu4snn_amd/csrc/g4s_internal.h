@@ -25,6 +25,11 @@ constexpr uint32_t kSentinel = 0xFFFFFFFFu;   // empty spike-queue slot
 constexpr int kCtlStride = 4;                 // per-step control words
 enum { kCtlCount = 0 };  // spikes of the step; k_ap keeps finished workgroups in word 1
 constexpr uint32_t kSpinLimit = 1u << 24;     // bounded device waits (~10 s)
+// Grid-barrier arrival counters: one per group of workgroups (blockIdx % 8,
+// the XCD a workgroup is dealt to), each in its own 256-byte line.
+constexpr int kBarGroups = 8;
+constexpr int kBarStride = 64;
+constexpr int kBarWords = kBarGroups * kBarStride;
 
 // Everything a kernel needs about one network, passed by value.
 //  - neuron state/parameters are structure-of-arrays fp32 (the reference's

@@ -176,30 +176,51 @@ __device__ __forceinline__ void push_flat(const DevNet &d, int t, int count, int
     }
 }
 
-// Grid-wide barrier for cooperative launches: monotonically increasing
-// arrival counter, target = epoch * gridDim.x.  Bounded: returns false (and
+// Grid-wide barrier for cooperative launches.  Workgroup b arrives on
+// counter b % 8 (its XCD group): device-scope atomics on one address are
+// serialised, so eight counters in separate lines take an eighth of the
+// arrivals each.  Counters only increase; at epoch e group g is complete when
+// its counter reaches e * (number of workgroups in g).  Lanes 0..7 of the
+// first wave poll the eight counters together.  Bounded: returns false (and
 // raises the error flag) if the wait exceeds kSpinLimit or another
 // workgroup raised the flag.
+// It orders only data written and read with agent-scope atomics (the spike
+// bitmap of the SKL kernel): every wave waits for its own stores to complete
+// before the workgroup arrives, and the loads after the barrier are issued
+// only once the final arrival has been seen.  It pays no release (an L2
+// write-back of the XCD) and no acquire (an L2 invalidate, which would also
+// evict the static synapse rows that the next step reads).
 __device__ bool grid_sync(const DevNet &d, uint32_t &epoch)
 {
     __shared__ int s_ok;
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's stores have completed
     __syncthreads();
-    if (threadIdx.x == 0) {
-        epoch += 1;
-        const uint32_t target = epoch * gridDim.x;
-        __hip_atomic_fetch_add(d.bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    epoch += 1;
+    if (threadIdx.x < kWave) {
+        const int lane = (int)threadIdx.x;
+        const uint32_t G = gridDim.x;
+        if (lane == 0)
+            __hip_atomic_fetch_add(&d.bar[(blockIdx.x % kBarGroups) * kBarStride], 1u, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t members = G / kBarGroups + ((uint32_t)lane < G % kBarGroups ? 1u : 0u);
+        const uint32_t target = epoch * members;
+        const uint32_t *ctr = &d.bar[(lane < kBarGroups ? lane : 0) * kBarStride];
         int ok = 1;
         uint32_t spins = 0;
-        // Relaxed polling: an acquire load would invalidate the caches on
-        // every iteration while other workgroups still compute.  One acquire
-        // fence after the wait orders the reads that follow.
-        while (__hip_atomic_load(d.bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            if (error_set(d)) { ok = 0; break; }
-            if (++spins > kSpinLimit) { raise_error(d); ok = 0; break; }
+        for (;;) {
+            bool reached = true;
+            if (lane < kBarGroups) reached = __hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target;
+            if (__all(reached)) break;
+            if (lane == 0 && error_set(d)) ok = 0;
+            if (!__shfl(ok, 0)) break;
+            if (++spins > kSpinLimit) {
+                if (lane == 0) raise_error(d);
+                ok = 0;
+                break;
+            }
             __builtin_amdgcn_s_sleep(2);
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        s_ok = ok;
+        if (lane == 0) s_ok = ok;
     }
     __syncthreads();
     return s_ok != 0;
@@ -414,10 +435,21 @@ __device__ __forceinline__ int slice_of_block(int b, int g)
     return (g % 8 == 0) ? (b % 8) * (g / 8) + b / 8 : b;
 }
 
+// Bitmap word w; kCoh: an agent-scope atomic load.
+template <bool kCoh, class P>
+__device__ __forceinline__ uint32_t bm_word(P bm, int w)
+{
+    if constexpr (kCoh)
+        return __hip_atomic_load(&bm[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+        return bm[w];
+}
+
 // Bitmap and raster words of one wave covering [wave_base, wave_base+64)
 // of a slice kernel; returns the wave's spike mask.  Waves starting at or
-// beyond `limit` write nothing.
-template <class Net>
+// beyond `limit` write nothing.  kCoh: bitmap words are stored with
+// agent-scope atomics (read by other workgroups of the same launch).
+template <bool kCoh = false, class Net>
 __device__ __forceinline__ uint64_t emit_bits(const Net &d, int t, int rel, int wave_base, bool spk, int limit)
 {
     const uint64_t mask = __ballot(spk);
@@ -427,7 +459,10 @@ __device__ __forceinline__ uint64_t emit_bits(const Net &d, int t, int rel, int 
     const int w = (wave_base >> 5) + lane;
     if (lane < 2 && w < d.words) {
         const uint32_t bits = lane ? (uint32_t)(mask >> 32) : (uint32_t)mask;
-        bm[w] = bits;
+        if (kCoh)
+            __hip_atomic_store(&bm[w], bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else
+            bm[w] = bits;
         if (d.raster) d.raster[(size_t)rel * (size_t)d.words + (size_t)w] = bits;
     }
     return mask;
@@ -604,7 +639,9 @@ __device__ __forceinline__ void owner_scan(uint16_t *own, int *s_wave, uint32_t 
 // (source, target) order; each window of them is counting-sorted by target
 // in LDS (count, scan, place) and each target then adds its entries in key
 // order, so the sum never depends on the order of the LDS atomics.
-template <bool kDet, int kEPT, int kSPT, bool kOwner, class Net>
+// kCoh: bitmap words are read with agent-scope atomics (written by other
+// workgroups of the same launch).
+template <bool kDet, int kEPT, int kSPT, bool kOwner, bool kCoh = false, class Net>
 __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int lo, int len, float *dyn)
 {
     const int B = (int)blockDim.x;
@@ -644,10 +681,10 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
     int mine = 0;
 #pragma unroll
     for (int k = 0; k < kKeep; ++k) {
-        kept[k] = (wb + k < we) ? bm[wb + k] : 0u;
+        kept[k] = (wb + k < we) ? bm_word<kCoh>(bm, wb + k) : 0u;
         mine += __popc(kept[k]);
     }
-    for (int w = wb + kKeep; w < we; ++w) mine += __popc(bm[w]);
+    for (int w = wb + kKeep; w < we; ++w) mine += __popc(bm_word<kCoh>(bm, w));
     int total = 0;
     const int first = block_exclusive_scan(mine, scratch, &total);
     const uint32_t stride = (uint32_t)d.P + 1u;
@@ -668,7 +705,7 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
             }
         }
         for (int w = wb + kKeep; w < we && idx < cap; ++w) {
-            uint32_t bits = bm[w];
+            uint32_t bits = bm_word<kCoh>(bm, w);
             while (bits) {
                 if (idx < cap) ids[idx] = (uint32_t)w * 32u + (uint32_t)__builtin_ctz(bits);
                 bits &= bits - 1u;
@@ -680,7 +717,7 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
         if (pass > 0 && mine > 0 && first < pass + cap && first + mine > pass) {
             int idx = first;
             for (int w = wb; w < we && idx < pass + cap; ++w) {
-                uint32_t bits = bm[w];
+                uint32_t bits = bm_word<kCoh>(bm, w);
                 while (bits) {
                     if (idx >= pass && idx < pass + cap) ids[idx - pass] = (uint32_t)w * 32u + (uint32_t)__builtin_ctz(bits);
                     bits &= bits - 1u;
@@ -1158,7 +1195,7 @@ __global__ void __launch_bounds__(1024) k_skl_slice(DevNet d, int t0, int steps,
     uint32_t epoch = 0;
     for (int rel = 0; rel < steps; ++rel) {
         const int t = t0 + rel;
-        slice_accumulate<kDet, kEntriesSingle, kSpikesSingle, kOwnerSingle>(d, t, p, lo, len, dyn);
+        slice_accumulate<kDet, kEntriesSingle, kSpikesSingle, kOwnerSingle, true>(d, t, p, lo, len, dyn);
         uint64_t masks[R];
 #pragma unroll
         for (int k = 0; k < R; ++k) {
@@ -1175,7 +1212,7 @@ __global__ void __launch_bounds__(1024) k_skl_slice(DevNet d, int t0, int steps,
                 nos[k] += spk ? 1 : 0;
                 if (t == last_t) d.lastI[j] = I;
             }
-            masks[k] = emit_bits(d, t, rel, lo + k * B + (tid & ~(kWave - 1)), spk, lo + len);
+            masks[k] = emit_bits<true>(d, t, rel, lo + k * B + (tid & ~(kWave - 1)), spk, lo + len);
         }
         slice_count<R>(d, rel, p, ntile, masks, &s_cnt);
         if (!grid_sync(d, epoch)) break;

@@ -422,7 +422,7 @@ int g4s_create(const g4s_spec *spec, g4s_net **out)
     if ((rc = net->alloc(&d.bm1, (size_t)d.words))) return bail(rc);
     if ((rc = net->alloc(&d.q0, n))) return bail(rc);
     if ((rc = net->alloc(&d.q1, n))) return bail(rc);
-    if ((rc = net->alloc(&d.bar, 1))) return bail(rc);
+    if ((rc = net->alloc(&d.bar, kBarWords))) return bail(rc);
     if ((rc = net->alloc(&d.err, 1))) return bail(rc);
     int32_t *tinfo = nullptr;
     if ((rc = net->alloc(&tinfo, 2))) return bail(rc);
@@ -652,7 +652,7 @@ int run_impl(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, 
         const int t0 = (int)(net->t);
         const int buf = chunk_no & 1;
         HIP_OK(launch_fill_u32((uint32_t *)d.ctl, (size_t)cs * kCtlStride, 0u, net->stream));
-        HIP_OK(launch_fill_u32(d.bar, 1, 0u, net->stream));
+        HIP_OK(launch_fill_u32(d.bar, kBarWords, 0u, net->stream));
         if (alg == G4S_ALG_AB && slice) {
             int32_t *ti = net->h_tinfo + 2 * buf;
             ti[0] = t0;
