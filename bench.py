@@ -286,8 +286,8 @@ def main():
     ap.add_argument("--seed", type=int, default=20221)
     ap.add_argument("--regime", type=int, default=1)
     # Default plan: what g4s_autotune picks for this bucket on MI355X
-    # (AB atomic, 256 slices, 1024 threads); --alg auto re-tunes in-process.
-    ap.add_argument("--alg", default="AB", help="scheme, or 'auto' for the autotuner's pick")
+    # (SKL atomic, 256 slices, 1024 threads); --alg auto re-tunes in-process.
+    ap.add_argument("--alg", default="SKL", help="scheme, or 'auto' for the autotuner's pick")
     ap.add_argument("--accum", default="atomic")
     ap.add_argument("--slices", type=int, default=256)
     ap.add_argument("--block", type=int, default=1024)
