@@ -93,6 +93,61 @@ __device__ __forceinline__ void emit(const DevNet &d, int t, int rel, int wave_b
     }
 }
 
+// Spike output of one workgroup tile [base, base + blockDim.x): each wave's
+// bitmap and raster words, and the tile's queue entries appended at one
+// device-scope atomic per workgroup.  Such atomics on one address are
+// serialised (about 30 ns each on MI355X), so one per wave would cost more.
+// Every thread of the workgroup calls it (it has barriers); at most 1024
+// threads.  kPublish (AP): the word ctl[kCtlCount..+1] is one 64-bit value,
+// spikes queued so far in the low half and finished workgroups in the high
+// half (`finished` adds one), and the slots are stored with agent-scope
+// atomics, because consumers of the same launch read them.
+template <bool kPublish>
+__device__ __forceinline__ void emit_tile(const DevNet &d, int t, int rel, int base, bool spk, int j, bool finished)
+{
+    __shared__ int s_wcnt[1024 / kWave];
+    __shared__ int s_base;
+    const int lane = lane_id();
+    const int wave = (int)threadIdx.x / kWave;
+    const int nwave = (int)blockDim.x / kWave;
+    const int wave_base = base + wave * kWave;
+    const uint64_t mask = __ballot(spk);
+    if (wave_base < d.n) {
+        uint32_t *bm = (t & 1) ? d.bm1 : d.bm0;
+        const int w = (wave_base >> 5) + lane;
+        if (lane < 2 && w < d.words) {
+            const uint32_t bits = lane ? (uint32_t)(mask >> 32) : (uint32_t)mask;
+            bm[w] = bits;
+            if (d.raster) d.raster[(size_t)rel * (size_t)d.words + (size_t)w] = bits;
+        }
+    }
+    if (lane == 0) s_wcnt[wave] = __popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int k = 0; k < nwave; ++k) tot += s_wcnt[k];
+        int32_t *ctl = d.ctl + rel * kCtlStride + kCtlCount;
+        if (kPublish) {
+            const uint64_t inc = (uint64_t)tot + (finished ? (1ull << 32) : 0ull);
+            s_base = (int)(uint32_t)__hip_atomic_fetch_add(reinterpret_cast<uint64_t *>(ctl), inc, __ATOMIC_RELAXED,
+                                                           __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            s_base = tot ? atomicAdd(ctl, tot) : 0;
+        }
+    }
+    __syncthreads();
+    if (spk) {
+        int pos = s_base + bits_below(mask);
+        for (int k = 0; k < wave; ++k) pos += s_wcnt[k];
+        uint32_t *q = (t & 1) ? d.q1 : d.q0;
+        if (kPublish)
+            __hip_atomic_store(&q[pos], (uint32_t)j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else
+            q[pos] = (uint32_t)j;
+    }
+    __syncthreads();  // s_wcnt and s_base are rewritten by the next tile
+}
+
 // Synaptic input of neuron j at step t in atomic mode: read and clear.
 __device__ __forceinline__ float take_input(const DevNet &d, int t, int j)
 {
@@ -267,7 +322,7 @@ __global__ void __launch_bounds__(256) k_update(DevNet d, int t, int rel, int la
         const int j = base + (int)threadIdx.x;
         bool spk = false;
         if (j < d.n) spk = neuron_step(d, t, j, take_input(d, t, j), last != 0);
-        emit(d, t, rel, base + ((int)threadIdx.x & ~(kWave - 1)), spk, j, d.n);
+        emit_tile<false>(d, t, rel, base, spk, j, false);
     }
 }
 
@@ -340,10 +395,8 @@ __global__ void __launch_bounds__(256) k_ap(DevNet d, int t, int rel, int last, 
         const int used = t == 0 ? 0 : (rel ? d.ctl[(rel - 1) * kCtlStride + kCtlCount] : prev_count);
         for (int i = blockIdx.x * blockDim.x + (int)threadIdx.x; i < used; i += stride) qn[i] = kSentinel;
     }
-    uint64_t *word = reinterpret_cast<uint64_t *>(d.ctl + rel * kCtlStride + kCtlCount);
-    uint32_t *q = (t & 1) ? d.q1 : d.q0;
-    __shared__ int s_wcnt[256 / kWave];
-    __shared__ int s_base;
+    const uint64_t *word = reinterpret_cast<const uint64_t *>(d.ctl + rel * kCtlStride + kCtlCount);
+    const uint32_t *q = (t & 1) ? d.q1 : d.q0;
     const int lane = lane_id();
     const int wave = (int)threadIdx.x / kWave;
     const int nwave = (int)blockDim.x / kWave;
@@ -351,32 +404,7 @@ __global__ void __launch_bounds__(256) k_ap(DevNet d, int t, int rel, int last, 
         const int j = base + (int)threadIdx.x;
         bool spk = false;
         if (j < d.n) spk = neuron_step(d, t, j, take_input(d, t, j), last != 0);
-        const int wave_base = base + wave * kWave;
-        const uint64_t mask = __ballot(spk);
-        if (wave_base < d.n) {
-            uint32_t *bm = (t & 1) ? d.bm1 : d.bm0;
-            const int w = (wave_base >> 5) + lane;
-            if (lane < 2 && w < d.words) {
-                const uint32_t bits = lane ? (uint32_t)(mask >> 32) : (uint32_t)mask;
-                bm[w] = bits;
-                if (d.raster) d.raster[(size_t)rel * (size_t)d.words + (size_t)w] = bits;
-            }
-        }
-        if (lane == 0) s_wcnt[wave] = __popcll(mask);
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int tot = 0;
-            for (int k = 0; k < nwave; ++k) tot += s_wcnt[k];
-            const uint64_t inc = (uint64_t)tot + (base + stride >= d.n ? (1ull << 32) : 0ull);
-            s_base = (int)(uint32_t)__hip_atomic_fetch_add(word, inc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        __syncthreads();
-        if (spk) {
-            int pos = s_base + bits_below(mask);
-            for (int k = 0; k < wave; ++k) pos += s_wcnt[k];
-            __hip_atomic_store(&q[pos], (uint32_t)j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        __syncthreads();  // s_wcnt and s_base are rewritten by the next tile
+        emit_tile<true>(d, t, rel, base, spk, j, base + stride >= d.n);
     }
 
     float *out = ((t + 1) & 1) ? d.isyn1 : d.isyn0;
@@ -1237,7 +1265,7 @@ __global__ void __launch_bounds__(256) k_pull_thread(DevNet d, int t, int rel, i
         const int j = base + (int)threadIdx.x;
         bool spk = false;
         if (j < d.n) spk = neuron_step(d, t, j, gather_input(d, t, j), last != 0);
-        emit(d, t, rel, base + ((int)threadIdx.x & ~(kWave - 1)), spk, j, d.n);
+        emit_tile<false>(d, t, rel, base, spk, j, false);
     }
 }
 
