@@ -47,7 +47,9 @@ def traffic_of(key):
             rec = json.load(f).get(key)
     except (OSError, ValueError):
         return None
-    return None if rec is None else rec["bytes_per_launch"]
+    if rec is None:
+        return None
+    return rec["bytes_per_step"] if "bytes_per_step" in rec else rec["bytes_per_launch"]
 
 
 def kernel_bytes(alg, accum, n, nsyn, spikes):
@@ -112,7 +114,7 @@ def bench_batch(g, args):
               f"{sorted(tuned.items())}", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     b = g.Batch(args.accum, chunk=args.chunk, slice_len=args.slice_len, schedule=args.schedule,
-                block=args.batch_block)
+                block=args.batch_block, streams=args.streams)
     slots = [b.add(n, ns, sd, total, thal_scale=sc, keep_raster=False) for (n, ns, sd, sc) in specs]
     print(f"batch of {len(slots)} built in {time.perf_counter() - t0:.1f} s, "
           f"{sum(n for n, _, _, _ in specs)} neurons, {sum(n * ns for n, ns, _, _ in specs) / 1e9:.2f}e9 synapses",
@@ -161,7 +163,7 @@ def bench_batch(g, args):
                    "accum": args.accum, "chunk": args.chunk,
                    "slice_len": args.slice_len or ("tuned per bucket" if tuned else "min(512, N)"),
                    "tuned": {f"log2N={k[0]},Nsyn={k[1]}": v for k, v in sorted(tuned.items())} or None,
-                   "schedule": args.schedule, "block": args.batch_block or 256,
+                   "schedule": args.schedule, "block": args.batch_block or 256, "streams": args.streams or 3,
                    "neurons": sum(x[0] for x in specs),
                    "synapses": sum(x[0] * x[1] for x in specs), "seed": args.seed},
         "events_per_s": events / wall,
@@ -169,11 +171,16 @@ def bench_batch(g, args):
         "speedup_vs_plain": (sim_s / wall) / (p_sim / p_wall),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic_of(f"k_batch_step|batch{len(specs)}|{args.accum}|chunk={args.chunk}|slice_len=512")
+                     "traffic": traffic_of(f"k_batch_step|batch{len(specs)}|{args.accum}|chunk={args.chunk}|slice_len=512"
+                                           f"|streams={args.streams or 3}|per_step")
                      if not args.slice_len and not args.tune and args.schedule == "step" and args.seed == 20221 else None,
                      "kernel": "k_batch_step",
-                     "bytes_per_launch": byts / max(1, launches),
-                     "avg_launch_us": dev_ms * 1e3 / max(1, launches)},
+                     # each step is one launch per instance group, the groups'
+                     # launches running concurrently on their own streams:
+                     # bytes and device time are per step of the whole batch
+                     "bytes_per_step": byts / max(1, steps_timed),
+                     "device_us_per_step": dev_ms * 1e3 / max(1, steps_timed),
+                     "launches_per_step": launches / max(1, steps_timed)},
     }
 
 
@@ -307,6 +314,8 @@ def main():
     ap.add_argument("--slice-len", type=int, default=0, help="batched: neurons per slice, 0 = per instance")
     ap.add_argument("--schedule", default="step", choices=["step", "queue"])
     ap.add_argument("--batch-block", type=int, default=0, help="batched: threads per workgroup (256/512/1024)")
+    ap.add_argument("--streams", type=int, default=0,
+                    help="batched: instance groups stepped on their own streams, 0 = the library default")
     ap.add_argument("--plain-sample", type=int, default=32)
     ap.add_argument("--tune", action="store_true", help="batched: tune the slice length per (N, Nsyn) bucket first")
     ap.add_argument("--no-extras", action="store_true",

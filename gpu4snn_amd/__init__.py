@@ -59,7 +59,7 @@ class Timing(C.Structure):
 
 class BatchOpts(C.Structure):
     _fields_ = [("chunk", C.c_int32), ("slice_len", C.c_int32), ("graph", C.c_int32), ("schedule", C.c_int32),
-                ("budget", C.c_int64), ("block", C.c_int32), ("reserved", C.c_int32)]
+                ("budget", C.c_int64), ("block", C.c_int32), ("streams", C.c_int32)]
 
 
 class BatchPlan(C.Structure):
@@ -335,13 +335,13 @@ class Batch:
 
     SCHEDULES = {"step": 0, "queue": 1}
 
-    def __init__(self, accum="det", chunk=0, slice_len=0, graph=0, budget=0, schedule="step", block=0):
+    def __init__(self, accum="det", chunk=0, slice_len=0, graph=0, budget=0, schedule="step", block=0, streams=0):
         """schedule "step": one launch per step over all instances
         (graph-replayed unless graph < 0); "queue": one persistent launch per
         chunk fed by a device work queue."""
         self._h = _vp()
         self.accum = accum
-        opts = BatchOpts(chunk, slice_len, graph, self.SCHEDULES[schedule], int(budget), block, 0)
+        opts = BatchOpts(chunk, slice_len, graph, self.SCHEDULES[schedule], int(budget), block, streams)
         _check(lib().g4s_batch_create(ACCUM[accum], C.byref(opts), C.byref(self._h)))
 
     def close(self):

@@ -31,6 +31,7 @@ namespace {
 
 constexpr int kBlock = 256;          // threads per workgroup of the batched kernels
 constexpr int kDefaultChunk = 100;
+constexpr int kMaxGroups = 4;         // schedule 0: instance groups on their own streams
 constexpr int kMaxSliceLen = 4096;
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
@@ -90,6 +91,11 @@ struct g4s_batch {
     int block = kBlock;  // threads per workgroup of the per-step kernel
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // schedule 0: instance groups; group 0 runs on `stream`, group g >= 1 on
+    // side[g-1], forked after the chunk's uploads and joined before its copies
+    int groups = 3;
+    hipStream_t side[kMaxGroups - 1] = {};
+    hipEvent_t fork = nullptr, join[kMaxGroups - 1] = {};
     std::vector<Inst> inst;
     int64_t bytes_in_use = 0;
     // descriptor table and its pinned mirror
@@ -116,10 +122,10 @@ struct g4s_batch {
     size_t done_cap = 0;
     int chunk_grid = 0;
     size_t chunk_grid_lds = 0;
-    // graph of one chunk
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    int g_cs = 0, g_grid = 0;
+    // graph of one chunk, one per group
+    hipGraph_t graph[kMaxGroups] = {};
+    hipGraphExec_t exec[kMaxGroups] = {};
+    int g_cs = 0, g_groups = 0, g_grid[kMaxGroups] = {};
     size_t g_lds = 0;
     int64_t gen = 0, g_gen = -1;
     g4s_timing timing{};
@@ -160,10 +166,13 @@ int grow_int(g4s_batch *b, int32_t **d, int32_t **h, int *cap, int need)
 
 void drop_graph(g4s_batch *b)
 {
-    if (b->exec) hipGraphExecDestroy(b->exec);
-    if (b->graph) hipGraphDestroy(b->graph);
-    b->exec = nullptr;
-    b->graph = nullptr;
+    for (int g = 0; g < kMaxGroups; ++g) {
+        if (b->exec[g]) hipGraphExecDestroy(b->exec[g]);
+        if (b->graph[g]) hipGraphDestroy(b->graph[g]);
+        b->exec[g] = nullptr;
+        b->graph[g] = nullptr;
+    }
+    b->g_groups = 0;
 }
 
 int check_slot(const g4s_batch *b, int32_t slot)
@@ -235,6 +244,7 @@ int g4s_batch_create(int accum, const g4s_batch_opts *opts, g4s_batch **out)
     if (o.block != 0 && o.block != 256 && o.block != 512 && o.block != 1024)
         return fail(G4S_ERR_ARG, "block must be 0, 256, 512 or 1024");
     if (o.schedule == 1 && o.block > 256) return fail(G4S_ERR_ARG, "the queue schedule runs 256-thread workgroups");
+    if (o.streams < 0 || o.streams > kMaxGroups) return fail(G4S_ERR_ARG, "streams must be 0..%d", kMaxGroups);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(G4S_ERR_HIP, "no HIP device");
     g4s_batch *b = new g4s_batch();
@@ -242,8 +252,14 @@ int g4s_batch_create(int accum, const g4s_batch_opts *opts, g4s_batch **out)
     b->opts = o;
     b->chunk = o.chunk > 0 ? o.chunk : kDefaultChunk;
     b->block = o.block > 0 ? o.block : kBlock;
-    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
+    b->groups = o.schedule == 1 ? 1 : (o.streams > 0 ? o.streams : 3);
+    bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreate(&b->ev0) == hipSuccess && hipEventCreate(&b->ev1) == hipSuccess;
+    if (ok && b->groups > 1) ok = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess;
+    for (int g = 1; ok && g < b->groups; ++g)  // default priority, like `stream`
+        ok = hipStreamCreateWithFlags(&b->side[g - 1], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&b->join[g - 1], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
         g4s_batch_destroy(b);
         return fail(G4S_ERR_HIP, "stream/event creation failed");
     }
@@ -389,42 +405,60 @@ int g4s_batch_step(g4s_batch *b, int32_t *active)
         d.last_t = (int32_t)(in.steps - 1);
         b->h_desc[s] = d;
     }
-    // work list.  Workgroup `blk` runs on XCD blk % 8.  Every instance goes
-    // whole to one XCD (its slices share that XCD's L2: the bitmap and the
-    // neighbouring segments of each row), instances are dealt to XCDs
-    // longest-first to the least loaded one (cost ~ slices * L * Nsyn), and
-    // each XCD runs its costlier slices first.
+    // work lists.  Schedule 0 splits the instances into groups of about
+    // equal cost (longest first, each to the cheapest group); each group is
+    // stepped by its own chain of launches on its own stream, so while one
+    // group's step drains its last workgroups another group's step fills the
+    // freed CUs.  Within a group, workgroup `blk` runs on XCD blk % 8.  Every
+    // instance goes whole to one XCD (its slices share that XCD's L2: the
+    // bitmap and the neighbouring segments of each row), instances are dealt
+    // to XCDs longest-first to the least loaded one (cost ~ slices * L *
+    // Nsyn), and each XCD runs its costlier slices first.
     std::vector<int32_t> order(act);
     auto cost = [&](int32_t s) {
         const DevNet &d = b->inst[(size_t)s].d;
         return (double)d.P * ((double)d.L * 8.0 + (double)d.L * (double)d.nsyn * 0.02);
     };
     std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return cost(x) > cost(y); });
-    std::vector<uint2> xcd[8];
-    double load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const int G = b->opts.schedule == 1 ? 1 : std::max(1, std::min<int>(b->groups, (int)act.size()));
+    std::vector<int32_t> grp[kMaxGroups];
+    double gload[kMaxGroups] = {};
     for (int32_t s : order) {
-        const int x = (int)(std::min_element(load, load + 8) - load);
-        load[x] += cost(s);
-        for (int p = 0; p < b->inst[(size_t)s].d.P; ++p) xcd[x].push_back(make_uint2((uint32_t)s, (uint32_t)p));
+        const int g = (int)(std::min_element(gload, gload + G) - gload);
+        gload[g] += cost(s);
+        grp[g].push_back(s);
     }
-    size_t per = 0;
-    for (auto &v : xcd) per = std::max(per, v.size());
-    const int grid = (int)std::max<size_t>(64, (per * 8 + 63) / 64 * 64);
+    int ggrid[kMaxGroups] = {}, goff[kMaxGroups] = {};
+    std::vector<uint2> wl;
+    for (int g = 0; g < G; ++g) {
+        std::vector<uint2> xcd[8];
+        double load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int32_t s : grp[g]) {
+            const int x = (int)(std::min_element(load, load + 8) - load);
+            load[x] += cost(s);
+            for (int p = 0; p < b->inst[(size_t)s].d.P; ++p) xcd[x].push_back(make_uint2((uint32_t)s, (uint32_t)p));
+        }
+        size_t per = 0;
+        for (auto &v : xcd) per = std::max(per, v.size());
+        ggrid[g] = (int)std::max<size_t>(64, (per * 8 + 63) / 64 * 64);
+        goff[g] = (int)wl.size();
+        for (int blk = 0; blk < ggrid[g]; ++blk) {
+            const std::vector<uint2> &v = xcd[blk % 8];
+            const size_t q = (size_t)(blk / 8);
+            wl.push_back(q < v.size() ? v[q] : make_uint2(kNoWork, 0u));
+        }
+    }
+    const int grid = (int)wl.size();
     size_t wc = (size_t)b->work_cap;
     if ((rc = grow_dev(b, &b->d_work, &b->h_work, &wc, (size_t)grid))) return rc;
     b->work_cap = (int)wc;
-    for (int blk = 0; blk < grid; ++blk) {
-        const std::vector<uint2> &v = xcd[blk % 8];
-        const size_t q = (size_t)(blk / 8);
-        b->h_work[blk] = q < v.size() ? v[q] : make_uint2(kNoWork, 0u);
-    }
+    std::copy(wl.begin(), wl.end(), b->h_work);
     for (size_t k = 0; k < act.size(); ++k) b->h_slots[k] = act[k];
     HIP_OK(hipMemcpyAsync(b->d_desc, b->h_desc, b->inst.size() * sizeof(DevNet), hipMemcpyHostToDevice, b->stream));
     HIP_OK(hipMemcpyAsync(b->d_work, b->h_work, (size_t)grid * sizeof(uint2), hipMemcpyHostToDevice, b->stream));
     HIP_OK(hipMemcpyAsync(b->d_slots, b->h_slots, act.size() * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
 
     const size_t lds = batch_lds(Lmax, det, b->block);
-    const Launch L{grid, b->block, b->stream};
     int launch_grid = grid;
     if (b->opts.schedule == 1) {
         // queue: the first step of every instance, costlier instances first
@@ -464,31 +498,55 @@ int g4s_batch_step(g4s_batch *b, int32_t *active)
         HIP_OK(hipEventRecord(b->ev0, b->stream));
         HIP_OK(launch_batch_chunk(b->d_desc, b->d_units, b->d_ctr, b->d_done, (int)total, det, lds,
                                   Launch{launch_grid, kBlock, b->stream}));
-    } else if (b->opts.graph < 0) {
-        HIP_OK(hipEventRecord(b->ev0, b->stream));
-        for (int rel = 0; rel < cs; ++rel) HIP_OK(launch_batch_step(b->d_desc, b->d_work, grid, rel, det, lds, L));
     } else {
+        auto st = [&](int g) { return g == 0 ? b->stream : b->side[g - 1]; };
         HIP_OK(hipEventRecord(b->ev0, b->stream));
-        if (!b->exec || b->g_cs != cs || b->g_grid != grid || b->g_lds != lds || b->g_gen != b->gen) {
-            drop_graph(b);
-            HIP_OK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-            hipError_t e = hipSuccess;
-            for (int rel = 0; rel < cs && e == hipSuccess; ++rel)
-                e = launch_batch_step(b->d_desc, b->d_work, grid, rel, det, lds, L);
-            hipGraph_t g = nullptr;
-            hipError_t e2 = hipStreamEndCapture(b->stream, &g);
-            if (e != hipSuccess || e2 != hipSuccess) {
-                if (g) hipGraphDestroy(g);
-                return fail(G4S_ERR_HIP, "batch graph capture: %s", hipGetErrorString(e != hipSuccess ? e : e2));
-            }
-            b->graph = g;
-            HIP_OK(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
-            b->g_cs = cs;
-            b->g_grid = grid;
-            b->g_lds = lds;
-            b->g_gen = b->gen;
+        if (G > 1) {  // the other groups start after this chunk's uploads
+            HIP_OK(hipEventRecord(b->fork, b->stream));
+            for (int g = 1; g < G; ++g) HIP_OK(hipStreamWaitEvent(st(g), b->fork, 0));
         }
-        HIP_OK(hipGraphLaunch(b->exec, b->stream));
+        if (b->opts.graph < 0) {
+            for (int rel = 0; rel < cs; ++rel)
+                for (int g = 0; g < G; ++g)
+                    HIP_OK(launch_batch_step(b->d_desc, b->d_work + goff[g], ggrid[g], rel, det, lds,
+                                             Launch{ggrid[g], b->block, st(g)}));
+        } else {
+            bool same = b->g_groups == G && b->g_cs == cs && b->g_lds == lds && b->g_gen == b->gen;
+            for (int g = 0; same && g < G; ++g) same = b->g_grid[g] == ggrid[g];
+            if (!same) {
+                drop_graph(b);
+                for (int g = 0; g < G; ++g) {
+                    HIP_OK(hipStreamBeginCapture(st(g), hipStreamCaptureModeThreadLocal));
+                    hipError_t e = hipSuccess;
+                    for (int rel = 0; rel < cs && e == hipSuccess; ++rel)
+                        e = launch_batch_step(b->d_desc, b->d_work + goff[g], ggrid[g], rel, det, lds,
+                                              Launch{ggrid[g], b->block, st(g)});
+                    hipGraph_t gr = nullptr;
+                    hipError_t e2 = hipStreamEndCapture(st(g), &gr);
+                    if (e != hipSuccess || e2 != hipSuccess) {
+                        if (gr) hipGraphDestroy(gr);
+                        drop_graph(b);
+                        return fail(G4S_ERR_HIP, "batch graph capture: %s",
+                                    hipGetErrorString(e != hipSuccess ? e : e2));
+                    }
+                    b->graph[g] = gr;
+                    if (hipGraphInstantiate(&b->exec[g], gr, nullptr, nullptr, 0) != hipSuccess) {
+                        drop_graph(b);
+                        return fail(G4S_ERR_HIP, "batch graph instantiation failed");
+                    }
+                    b->g_grid[g] = ggrid[g];
+                }
+                b->g_groups = G;
+                b->g_cs = cs;
+                b->g_lds = lds;
+                b->g_gen = b->gen;
+            }
+            for (int g = 0; g < G; ++g) HIP_OK(hipGraphLaunch(b->exec[g], st(g)));
+        }
+        for (int g = 1; g < G; ++g) {  // join before the chunk's copies on `stream`
+            HIP_OK(hipEventRecord(b->join[g - 1], st(g)));
+            HIP_OK(hipStreamWaitEvent(b->stream, b->join[g - 1], 0));
+        }
     }
     HIP_OK(hipEventRecord(b->ev1, b->stream));
     if (b->opts.schedule == 1)
@@ -525,7 +583,7 @@ int g4s_batch_step(g4s_batch *b, int32_t *active)
     b->timing.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
     b->timing.steps = steps;
     b->timing.spikes = spikes;
-    b->timing.launches = b->opts.schedule == 1 ? 1 : cs;
+    b->timing.launches = b->opts.schedule == 1 ? 1 : cs * G;
     b->timing.grid = launch_grid;
     return G4S_OK;
 }
@@ -705,6 +763,8 @@ int g4s_batch_destroy(g4s_batch *b)
 {
     if (!b) return G4S_OK;
     if (b->stream) hipStreamSynchronize(b->stream);
+    for (hipStream_t st : b->side)
+        if (st) hipStreamSynchronize(st);
     drop_graph(b);
     for (Inst &in : b->inst)
         if (in.block) hipFree(in.block);
@@ -716,6 +776,11 @@ int g4s_batch_destroy(g4s_batch *b)
         if (p) hipHostFree(p);
     if (b->ev0) hipEventDestroy(b->ev0);
     if (b->ev1) hipEventDestroy(b->ev1);
+    if (b->fork) hipEventDestroy(b->fork);
+    for (int g = 0; g < kMaxGroups - 1; ++g) {
+        if (b->join[g]) hipEventDestroy(b->join[g]);
+        if (b->side[g]) hipStreamDestroy(b->side[g]);
+    }
     if (b->stream) hipStreamDestroy(b->stream);
     delete b;
     return G4S_OK;

@@ -202,7 +202,9 @@ typedef struct g4s_batch_opts {
                             the device as each instance's previous step completes */
     int64_t budget;      /* device bytes the batch may hold, 0 = no limit            */
     int32_t block;       /* schedule 0: threads per workgroup, 256 (0), 512 or 1024   */
-    int32_t reserved;    /* must be 0                                                */
+    int32_t streams;     /* schedule 0: the instances are split into this many groups
+                            of about equal cost, each stepped on its own stream, so
+                            one group's launches fill another's tail; 0 = 3, 1..4    */
 } g4s_batch_opts;
 
 typedef struct g4s_inst_info {

@@ -52,6 +52,25 @@ def test_batch_det_matches_oracle(g4s, oracle, chunk, graph, schedule):
             check_instance(g4s, oracle, b, slot, n, ns, sd, sc, st)
 
 
+@pytest.mark.parametrize("streams,graph", [(1, 0), (2, -1), (3, 0), (4, 0), (4, -1)])
+def test_batch_stream_groups_det(g4s, oracle, streams, graph):
+    """Instances split into groups, each group's launches on its own stream
+    (graph-replayed or launched directly): every instance still matches the
+    oracle bit for bit, also when a group is left with no instance (more
+    groups than the last chunk's unfinished instances)."""
+    with g4s.Batch("det", chunk=40, graph=graph, streams=streams) as b:
+        slots = [b.add(n, ns, sd, st, thal_scale=sc) for (n, ns, sd, sc, st) in SPECS]
+        b.run()
+        assert b.timing()["launches"] >= 1
+        for slot, (n, ns, sd, sc, st) in zip(slots, SPECS):
+            check_instance(g4s, oracle, b, slot, n, ns, sd, sc, st)
+
+
+def test_batch_stream_groups_invalid(g4s):
+    with pytest.raises(g4s.G4SError):
+        g4s.Batch("det", streams=5)
+
+
 @pytest.mark.parametrize("slice_len", [64, 192, 1024, 2048])
 def test_batch_det_slice_lengths(g4s, oracle, slice_len):
     specs = [(3000, 300, 21, 1.0, 120), (5000, 1000, 22, 1.3, 100)]
