@@ -703,7 +703,9 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
     const int W = d.words;
     const int per = (W + B - 1) / B;
     const int wb = min(tid * per, W), we = min(wb + per, W);
-    // up to kKeep words stay in registers for the id pass below
+    // up to kKeep words stay in registers for the id pass below; longer
+    // runs (large networks) are read kKeep words at a time, all of a group's
+    // loads in flight before the first wait
     constexpr int kKeep = 8;
     uint32_t kept[kKeep];
     int mine = 0;
@@ -712,7 +714,13 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
         kept[k] = (wb + k < we) ? bm_word<kCoh>(bm, wb + k) : 0u;
         mine += __popc(kept[k]);
     }
-    for (int w = wb + kKeep; w < we; ++w) mine += __popc(bm_word<kCoh>(bm, w));
+    for (int w0 = wb + kKeep; w0 < we; w0 += kKeep) {
+        uint32_t x[kKeep];
+#pragma unroll
+        for (int k = 0; k < kKeep; ++k) x[k] = (w0 + k < we) ? bm_word<kCoh>(bm, w0 + k) : 0u;
+#pragma unroll
+        for (int k = 0; k < kKeep; ++k) mine += __popc(x[k]);
+    }
     int total = 0;
     const int first = block_exclusive_scan(mine, scratch, &total);
     const uint32_t stride = (uint32_t)d.P + 1u;
@@ -732,24 +740,37 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
                 ++idx;
             }
         }
-        for (int w = wb + kKeep; w < we && idx < cap; ++w) {
-            uint32_t bits = bm_word<kCoh>(bm, w);
-            while (bits) {
-                if (idx < cap) ids[idx] = (uint32_t)w * 32u + (uint32_t)__builtin_ctz(bits);
-                bits &= bits - 1u;
-                ++idx;
+        for (int w0 = wb + kKeep; w0 < we && idx < cap; w0 += kKeep) {
+            uint32_t x[kKeep];
+#pragma unroll
+            for (int k = 0; k < kKeep; ++k) x[k] = (w0 + k < we) ? bm_word<kCoh>(bm, w0 + k) : 0u;
+#pragma unroll
+            for (int k = 0; k < kKeep; ++k) {
+                uint32_t bits = x[k];
+                while (bits) {
+                    if (idx < cap) ids[idx] = (uint32_t)(w0 + k) * 32u + (uint32_t)__builtin_ctz(bits);
+                    bits &= bits - 1u;
+                    ++idx;
+                }
             }
         }
     }
     for (int pass = 0; pass < total; pass += cap) {
         if (pass > 0 && mine > 0 && first < pass + cap && first + mine > pass) {
             int idx = first;
-            for (int w = wb; w < we && idx < pass + cap; ++w) {
-                uint32_t bits = bm_word<kCoh>(bm, w);
-                while (bits) {
-                    if (idx >= pass && idx < pass + cap) ids[idx - pass] = (uint32_t)w * 32u + (uint32_t)__builtin_ctz(bits);
-                    bits &= bits - 1u;
-                    ++idx;
+            for (int w0 = wb; w0 < we && idx < pass + cap; w0 += kKeep) {
+                uint32_t x[kKeep];
+#pragma unroll
+                for (int k = 0; k < kKeep; ++k) x[k] = (w0 + k < we) ? bm_word<kCoh>(bm, w0 + k) : 0u;
+#pragma unroll
+                for (int k = 0; k < kKeep; ++k) {
+                    uint32_t bits = x[k];
+                    while (bits) {
+                        if (idx >= pass && idx < pass + cap)
+                            ids[idx - pass] = (uint32_t)(w0 + k) * 32u + (uint32_t)__builtin_ctz(bits);
+                        bits &= bits - 1u;
+                        ++idx;
+                    }
                 }
             }
         }
@@ -898,15 +919,27 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
                 for (int k = 0; k < kEPT; ++k)
                     if (x[k].x != 0xFFFFFFFFu) dkey[atomicAdd(&dcur[(int)x[k].x - lo], 1)] = (uint16_t)(k * B + tid);
                 __syncthreads();
-                // each entry's rank in its bucket puts its weight in key order
+                // each entry's rank in its bucket puts its weight in key order;
+                // the bucket's keys are read eight at a time (16-byte aligned
+                // LDS reads; dkey is 16-byte aligned and Wd a multiple of 8)
 #pragma unroll
                 for (int k = 0; k < kEPT; ++k)
                     if (x[k].x != 0xFFFFFFFFu) {
                         const int j = (int)x[k].x - lo;
                         const int s0 = dstart[j], s1 = dcur[j];
-                        const uint32_t key = (uint32_t)(k * B + tid);
+                        const int key = k * B + tid;
                         int r = 0;
-                        for (int q = s0; q < s1; ++q) r += (uint32_t)dkey[q] < key ? 1 : 0;
+                        for (int q = s0 & ~7; q < s1; q += 8) {
+                            const uint4 kv = *(const uint4 *)(dkey + q);
+                            const uint32_t h[4] = {kv.x, kv.y, kv.z, kv.w};
+#pragma unroll
+                            for (int e = 0; e < 8; ++e) {
+                                const int kq = (int)((h[e >> 1] >> ((e & 1) * 16)) & 0xFFFFu);
+                                // position q+e in [s0, s1) and key below ours
+                                const bool in = (uint32_t)(q + e - s0) < (uint32_t)(s1 - s0);
+                                r += (in && kq < key) ? 1 : 0;
+                            }
+                        }
                         dw[s0 + r] = u2f(x[k].y);
                     }
                 __syncthreads();
