@@ -111,6 +111,9 @@ hipError_t launch_skl_slice(const DevNet &d, int t0, int steps, int last_t, bool
 int skl_slice_blocks_per_cu(int L, int block, bool det);
 // Dynamic LDS bytes of the slice kernels (single network at `block`
 // threads; batched at 256).
+// LDS of one workgroup on MI355X (gfx950): single-network slice kernels run
+// one workgroup per CU and may use all of it.
+constexpr size_t kSliceLdsMax = 160 * 1024;
 size_t slice_lds(int L, int block, bool det);
 size_t batch_lds(int L, bool det, int block = 256);
 hipError_t launch_pull_thread(const DevNet &d, int t, int rel, int last, const Launch &L);

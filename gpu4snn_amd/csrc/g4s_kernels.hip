@@ -543,7 +543,12 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int *s_tmp, int *tota
 
 // Spikes per pass per thread and entry loads in flight per thread:
 // single-network slices (up to 1024 threads) and batched slices (256).
+// Large single-network slices in atomic mode (kLargeSlice neurons or more:
+// networks of about 10^6 neurons) take four times the spikes per pass and
+// twice the entry loads in flight, cutting the dependent levels of a step
+// whose spikes need several passes.
 constexpr int kSpikesSingle = 2, kEntriesSingle = 4;
+constexpr int kSpikesLarge = 8, kEntriesLarge = 8, kLargeSlice = 2048;
 constexpr int kEntriesBatch = 8;
 // batched det slices keep 2 spikes per thread, which keeps their LDS
 // within 7 workgroups per CU at 512-neuron slices
@@ -575,6 +580,19 @@ __host__ __device__ inline size_t slice_lds_bytes(int L, int block, bool det, in
     if (!owner) b += (cap * 4 + 15) & ~(size_t)15;
     if (det) return b + wd * 6 + (size_t)L * 8;
     return b + (owner ? wd * 4 : 0);
+}
+
+// Single-network slice plans: large (atomic mode, slices of kLargeSlice
+// neurons or more whose workspace fits the LDS) or the default.
+__host__ inline bool large_slice_plan(int L, int block, bool det)
+{
+    return !det && L >= kLargeSlice &&
+           slice_lds_bytes(L, block, false, kEntriesLarge, kSpikesLarge, kOwnerSingle) <= kSliceLdsMax;
+}
+__host__ inline size_t single_slice_lds(int L, int block, bool det)
+{
+    return large_slice_plan(L, block, det) ? slice_lds_bytes(L, block, false, kEntriesLarge, kSpikesLarge, kOwnerSingle)
+                                           : slice_lds_bytes(L, block, det, kEntriesSingle, kSpikesSingle, kOwnerSingle);
 }
 
 // One {target, weight bits} table entry (generic or global pointer).
@@ -1003,7 +1021,7 @@ __global__ void __launch_bounds__(256) k_reduce_counts(DevNet d, int steps)
 
 // AB scheme: one launch per step (rel-th of the chunk; t = tinfo[0] + rel),
 // grid = P slices.  Thread state is loaded from and stored to HBM.
-template <int R, bool kDet>
+template <int R, bool kDet, bool kLarge>
 __global__ void __launch_bounds__(1024) k_slice_step(DevNet d, int rel)
 {
     extern __shared__ __align__(16) float dyn[];  // slice_accumulate needs 16-byte alignment
@@ -1031,7 +1049,8 @@ __global__ void __launch_bounds__(1024) k_slice_step(DevNet d, int rel)
             I0[k] = ((j < d.n_exc) ? d.ge : d.gi) * thalamic(d.seed, (uint32_t)t, (uint32_t)j);
         }
     }
-    slice_accumulate<kDet, kEntriesSingle, kSpikesSingle, kOwnerSingle>(d, t, p, lo, len, dyn);
+    slice_accumulate<kDet, kLarge ? kEntriesLarge : kEntriesSingle, kLarge ? kSpikesLarge : kSpikesSingle,
+                     kOwnerSingle>(d, t, p, lo, len, dyn);
     uint64_t masks[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
@@ -1227,7 +1246,7 @@ __global__ void __launch_bounds__(256) k_batch_counts(const DevNet *descs, const
 // Each thread keeps the state of its R neurons (j = lo + k*B + tid) in
 // registers for the whole chunk; per step: slice accumulation from the
 // previous step's region queue, update, spike output, one grid barrier.
-template <int R, bool kDet>
+template <int R, bool kDet, bool kLarge>
 __global__ void __launch_bounds__(1024) k_skl_slice(DevNet d, int t0, int steps, int last_t)
 {
     extern __shared__ __align__(16) float dyn[];  // slice_accumulate needs 16-byte alignment
@@ -1256,7 +1275,8 @@ __global__ void __launch_bounds__(1024) k_skl_slice(DevNet d, int t0, int steps,
     uint32_t epoch = 0;
     for (int rel = 0; rel < steps; ++rel) {
         const int t = t0 + rel;
-        slice_accumulate<kDet, kEntriesSingle, kSpikesSingle, kOwnerSingle, true>(d, t, p, lo, len, dyn);
+        slice_accumulate<kDet, kLarge ? kEntriesLarge : kEntriesSingle, kLarge ? kSpikesLarge : kSpikesSingle,
+                         kOwnerSingle, true>(d, t, p, lo, len, dyn);
         uint64_t masks[R];
 #pragma unroll
         for (int k = 0; k < R; ++k) {
@@ -1557,29 +1577,42 @@ hipError_t coop(const void *fn, const Launch &L, void **args, size_t shmem = 0)
     return hipLaunchCooperativeKernel(fn, dim3(L.grid), dim3(L.block), args, shmem, L.stream);
 }
 
+// The large plan exists in atomic mode for R >= 2 (slices of kLargeSlice
+// neurons or more at up to 1024 threads).
+template <int R, bool kDet>
+const void *skl_slice_fn_r(bool large)
+{
+    if constexpr (!kDet && R >= 2)
+        if (large) return (const void *)k_skl_slice<R, false, true>;
+    return (const void *)k_skl_slice<R, kDet, false>;
+}
+
 template <int R, bool kDet>
 hipError_t launch_skl_slice_r(const DevNet &d, int t0, int steps, int last_t, const Launch &L)
 {
     DevNet dd = d;
     void *args[] = {&dd, &t0, &steps, &last_t};
-    return coop((const void *)k_skl_slice<R, kDet>, L, args, slice_lds_bytes(d.L, L.block, kDet, kEntriesSingle, kSpikesSingle, kOwnerSingle));
+    return coop(skl_slice_fn_r<R, kDet>(large_slice_plan(d.L, L.block, kDet)), L, args,
+                single_slice_lds(d.L, L.block, kDet));
 }
 
 template <int R, bool kDet>
 hipError_t launch_slice_step_r(const DevNet &d, int rel, const Launch &L)
 {
-    return plain_launch(k_slice_step<R, kDet>, L, (uint32_t)slice_lds_bytes(d.L, L.block, kDet, kEntriesSingle, kSpikesSingle, kOwnerSingle), d,
-                        rel);
+    const uint32_t lds = (uint32_t)single_slice_lds(d.L, L.block, kDet);
+    if constexpr (!kDet && R >= 2)
+        if (large_slice_plan(d.L, L.block, kDet)) return plain_launch(k_slice_step<R, false, true>, L, lds, d, rel);
+    return plain_launch(k_slice_step<R, kDet, false>, L, lds, d, rel);
 }
 
 template <bool kDet>
-const void *skl_slice_fn(int per_thread)
+const void *skl_slice_fn(int per_thread, bool large)
 {
-    if (per_thread <= 1) return (const void *)k_skl_slice<1, kDet>;
-    if (per_thread <= 2) return (const void *)k_skl_slice<2, kDet>;
-    if (per_thread <= 4) return (const void *)k_skl_slice<4, kDet>;
-    if (per_thread <= 8) return (const void *)k_skl_slice<8, kDet>;
-    return (const void *)k_skl_slice<16, kDet>;
+    if (per_thread <= 1) return skl_slice_fn_r<1, kDet>(large);
+    if (per_thread <= 2) return skl_slice_fn_r<2, kDet>(large);
+    if (per_thread <= 4) return skl_slice_fn_r<4, kDet>(large);
+    if (per_thread <= 8) return skl_slice_fn_r<8, kDet>(large);
+    return skl_slice_fn_r<16, kDet>(large);
 }
 
 }  // namespace
@@ -1679,14 +1712,14 @@ int skl_slice_blocks_per_cu(int L, int block, bool det)
 {
     int per_cu = 0;
     const int r = (L + block - 1) / block;
-    const void *fn = det ? skl_slice_fn<true>(r) : skl_slice_fn<false>(r);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, slice_lds_bytes(L, block, det, kEntriesSingle, kSpikesSingle, kOwnerSingle)) !=
-        hipSuccess)
+    const bool large = large_slice_plan(L, block, det);
+    const void *fn = det ? skl_slice_fn<true>(r, large) : skl_slice_fn<false>(r, large);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, single_slice_lds(L, block, det)) != hipSuccess)
         return 0;
     return per_cu;
 }
 
-size_t slice_lds(int L, int block, bool det) { return slice_lds_bytes(L, block, det, kEntriesSingle, kSpikesSingle, kOwnerSingle); }
+size_t slice_lds(int L, int block, bool det) { return single_slice_lds(L, block, det); }
 size_t batch_lds(int L, bool det, int block)
 {
     return slice_lds_bytes(L, block, det, kEntriesBatch, det ? spikes_batch<true>() : spikes_batch<false>(), kOwnerBatch);

@@ -269,7 +269,7 @@ int64_t pick_slice_len(int64_t n, int64_t want)
 {
     int64_t P = std::max<int64_t>(1, std::min<int64_t>(want, (n + 63) / 64));
     int64_t L = ((n + P - 1) / P + 63) / 64 * 64;
-    return std::min<int64_t>(L, 8 * 1024);
+    return std::min<int64_t>(L, 16 * 1024);
 }
 
 int build_slices(g4s_net *net, int want)
@@ -281,9 +281,9 @@ int build_slices(g4s_net *net, int want)
     const int64_t L = pick_slice_len(n, want);
     const int64_t P = (n + L - 1) / L;
     if (net->slices_ready && net->d.L == L) return G4S_OK;
-    // LDS holds acc[L] plus the pass workspace (checked per run, at most
-    // 64 KB per workgroup)
-    if (L * 4 > 32 * 1024)
+    // LDS holds acc[L] plus the pass workspace (checked per run against
+    // the LDS of one workgroup); a thread updates at most 16 neurons
+    if (L > 16 * 1024)
         return fail(G4S_ERR_UNSUPPORTED, "N=%lld needs %lld slices of %lld: beyond the slice schemes' LDS", (long long)n,
                     (long long)P, (long long)L);
     for (void **pp : {(void **)&net->d.soff, (void **)&net->d.bcnt}) {
@@ -547,14 +547,14 @@ int run_impl(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, 
         for (int64_t want = std::max(net->cus, 1);; want *= 2) {
             if ((rc = build_slices(net, (int)want))) return rc;
             sblock = slice_block(net, o.block);
-            if (slice_lds(net->d.L, sblock, det) <= 64 * 1024 || net->d.L <= 64) break;
+            if (slice_lds(net->d.L, sblock, det) <= kSliceLdsMax || net->d.L <= 64) break;
         }
     }
     if (slice && (int64_t)net->d.L > 16 * (int64_t)sblock)
         return fail(G4S_ERR_ARG, "block %d too small for slices of %d neurons (max 16 per thread)", sblock, net->d.L);
-    if (slice && slice_lds(net->d.L, sblock, det) > 64 * 1024)
-        return fail(G4S_ERR_UNSUPPORTED, "slices of %d neurons at block %d need %zu bytes of LDS (max 65536)", net->d.L,
-                    sblock, slice_lds(net->d.L, sblock, det));
+    if (slice && slice_lds(net->d.L, sblock, det) > kSliceLdsMax)
+        return fail(G4S_ERR_UNSUPPORTED, "slices of %d neurons at block %d need %zu bytes of LDS (max %zu)", net->d.L,
+                    sblock, slice_lds(net->d.L, sblock, det), kSliceLdsMax);
 
     DevNet &d = net->d;
     Launch L{};

@@ -208,11 +208,11 @@ def test_autotune_plan_runs_and_is_cached(g4s, oracle):
 
 
 def test_default_tiling_fits_det_workspace(g4s, oracle):
-    """Default AB tiling at N=400k: one slice per CU would need more than
-    64 KB of det workspace per workgroup, so the runtime doubles the slice
-    count until it fits; the run stays bit-identical to the oracle, and
-    atomic mode keeps one slice per CU."""
-    n, nsyn, seed, steps = 400_000, 100, 5, 20
+    """Default AB tiling at N=4M: one slice per CU would need more than the
+    160 KB of LDS of one workgroup for det's workspace, so the runtime doubles
+    the slice count until it fits; the run stays bit-identical to the oracle,
+    and atomic mode keeps one slice per CU."""
+    n, nsyn, seed, steps = 4_000_000, 50, 5, 20
     onet = oracle.Network(n, nsyn, seed)
     oraster, ocounts, _ = onet.run(steps)
     with g4s.Network(n, nsyn, seed) as net:
@@ -225,5 +225,34 @@ def test_default_tiling_fits_det_workspace(g4s, oracle):
         net.reset()
         net.run(steps, alg="AB", accum="atomic")
         p_atomic = net.timing()["grid"]
-    # lengths round up to multiples of 64: 481 slices of 832 vs 250 of 1600
+    # lengths round up to multiples of 64: 509 slices of 7872 vs 256 of 15680
     assert p_det > 3 * p_atomic // 2 > 0, (p_det, p_atomic)
+
+
+@pytest.mark.parametrize("block", [256, 1024])
+def test_large_slices_both_modes(g4s, oracle, block):
+    """16 slices of 3136 neurons at N=50,000.  Atomic mode runs the large
+    plan (slices of 2048 neurons or more: 8 spikes per thread per pass, 8
+    entry loads in flight per thread); det mode at 1024 threads needs 86 KB of
+    LDS per workgroup.  Det is bit-identical to the oracle in the balanced and
+    irregular regimes.  Atomic mode keeps the oracle's raster for the first 10
+    steps (a lost or misplaced segment changes it at once) and the spike total
+    within 1%; in the irregular regime (about 1200 spikes per step) a step
+    needs two spike passes at 256 threads."""
+    n, nsyn, seed, steps = 50_000, 1000, 7, 300
+    for regime in (1, 2):
+        onet = oracle.Network(n, nsyn, seed, regime=regime)
+        oraster, ocounts, _ = onet.run(steps, threads=8)
+        for alg in ("AB", "SKL"):
+            with g4s.Network(n, nsyn, seed, regime=regime) as net:
+                words, counts = net.run(steps, alg=alg, accum="det", slices=16, block=block)
+                assert net.timing()["grid"] == 16
+                assert np.array_equal(g4s.raster_to_dense(words, n), oraster), (alg, block, regime)
+                v, _, _, _ = net.state()
+                assert np.array_equal(v.view(np.uint32), onet.v.view(np.uint32)), (alg, block, regime)
+                net.reset()
+                words, counts = net.run(steps, alg=alg, accum="atomic", slices=16, block=block)
+                assert net.timing()["grid"] == 16
+                d = g4s.raster_to_dense(words[:10], n)
+                assert np.array_equal(d, oraster[:10]), (alg, block, regime)
+                assert abs(int(counts.sum()) - int(ocounts.sum())) <= 0.01 * ocounts.sum(), (alg, block, regime)
