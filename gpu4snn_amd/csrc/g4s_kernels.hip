@@ -497,20 +497,23 @@ __device__ __forceinline__ uint64_t emit_bits(const Net &d, int t, int rel, int 
 }
 
 // Spike count of slice p at step t (waves' ballots of each tile) written to
-// bcnt[rel][p] for the host's per-step counts.  No global atomics.
+// bcnt[rel][p] for the host's per-step counts.  No global atomics.  *s_cnt
+// is zero on entry (zeroed before a workgroup barrier that precedes the
+// call) and again on return for the next step's call.
 template <int R>
 __device__ __forceinline__ void slice_count(const DevNet &d, int rel, int p, int ntile, const uint64_t (&masks)[R],
                                             int *s_cnt)
 {
-    if (threadIdx.x == 0) *s_cnt = 0;
-    __syncthreads();
     int c = 0;
 #pragma unroll
     for (int k = 0; k < R; ++k)
         if (k < ntile) c += __popcll(masks[k]);
     if (lane_id() == 0 && c) atomicAdd(s_cnt, c);
     __syncthreads();
-    if (threadIdx.x == 0) d.bcnt[(size_t)rel * (size_t)d.P + (size_t)p] = *s_cnt;
+    if (threadIdx.x == 0) {
+        d.bcnt[(size_t)rel * (size_t)d.P + (size_t)p] = *s_cnt;
+        *s_cnt = 0;
+    }
 }
 
 // Exclusive prefix sum over the workgroup of one value per thread; returns
@@ -989,7 +992,7 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
         }
         __syncthreads();
     }
-    __syncthreads();
+    if (total == 0) __syncthreads();  // otherwise the last pass ended with one
 }
 
 // Offset table: thread per (row i, boundary p): first index of row i whose
@@ -1042,6 +1045,7 @@ __global__ void __launch_bounds__(1024) k_slice_step(DevNet d, int rel)
     float *acc = dyn;
     const int t = d.tinfo[0] + rel;
     const bool last = t == d.tinfo[1];
+    if (threadIdx.x == 0) s_cnt = 0;  // slice_accumulate's barriers precede slice_count
     const int p = slice_of_block((int)blockIdx.x, (int)gridDim.x);
     const int lo = p * d.L;
     const int len = min(d.L, d.n - lo);
@@ -1286,6 +1290,7 @@ __global__ void __launch_bounds__(1024) k_skl_slice(DevNet d, int t0, int steps,
         }
     }
     uint32_t epoch = 0;
+    if (tid == 0) s_cnt = 0;  // slice_accumulate's barriers precede slice_count
     for (int rel = 0; rel < steps; ++rel) {
         const int t = t0 + rel;
         using Plan = SlicePlan<kDet, kLarge>;
