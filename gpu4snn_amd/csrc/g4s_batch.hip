@@ -139,8 +139,8 @@ int grow_dev(g4s_batch *b, T **d, T **h, size_t *cap, size_t need)
     if (need <= *cap) return G4S_OK;
     size_t want = std::max<size_t>(need, *cap + *cap / 2);
     want = std::max<size_t>(want, 64);
-    if (*d) hipFree(*d);
-    if (h && *h) hipHostFree(*h);
+    if (*d) (void)hipFree(*d);
+    if (h && *h) (void)hipHostFree(*h);
     *d = nullptr;
     if (h) *h = nullptr;
     if (hipMalloc((void **)d, want * sizeof(T)) != hipSuccess) {
@@ -167,8 +167,8 @@ int grow_int(g4s_batch *b, int32_t **d, int32_t **h, int *cap, int need)
 void drop_graph(g4s_batch *b)
 {
     for (int g = 0; g < kMaxGroups; ++g) {
-        if (b->exec[g]) hipGraphExecDestroy(b->exec[g]);
-        if (b->graph[g]) hipGraphDestroy(b->graph[g]);
+        if (b->exec[g]) (void)hipGraphExecDestroy(b->exec[g]);
+        if (b->graph[g]) (void)hipGraphDestroy(b->graph[g]);
         b->exec[g] = nullptr;
         b->graph[g] = nullptr;
     }
@@ -308,7 +308,7 @@ int g4s_batch_add(g4s_batch *b, const g4s_spec *spec, int64_t steps, int keep_ra
     d.P = (int32_t)P;
     d.L = L;
     if ((rc = build_synapses(spec, syn, b->stream))) {
-        hipFree(blk);
+        (void)hipFree(blk);
         return rc;
     }
     uint16_t *soff = (uint16_t *)(base + lay.soff);
@@ -316,7 +316,7 @@ int g4s_batch_add(g4s_batch *b, const g4s_spec *spec, int64_t steps, int keep_ra
     if (e == hipSuccess) e = launch_slice_offsets(d, soff, b->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
     if (e != hipSuccess) {
-        hipFree(blk);
+        (void)hipFree(blk);
         return fail(G4S_ERR_HIP, "instance initialisation: %s", hipGetErrorString(e));
     }
     d.soff = soff;
@@ -524,7 +524,7 @@ int g4s_batch_step(g4s_batch *b, int32_t *active)
                     hipGraph_t gr = nullptr;
                     hipError_t e2 = hipStreamEndCapture(st(g), &gr);
                     if (e != hipSuccess || e2 != hipSuccess) {
-                        if (gr) hipGraphDestroy(gr);
+                        if (gr) (void)hipGraphDestroy(gr);
                         drop_graph(b);
                         return fail(G4S_ERR_HIP, "batch graph capture: %s",
                                     hipGetErrorString(e != hipSuccess ? e : e2));
@@ -665,7 +665,7 @@ int g4s_batch_remove(g4s_batch *b, int32_t slot)
     if (rc) return rc;
     Inst &in = b->inst[(size_t)slot];
     HIP_OK(hipStreamSynchronize(b->stream));
-    if (in.block) hipFree(in.block);
+    if (in.block) (void)hipFree(in.block);
     b->bytes_in_use -= in.bytes;
     in = Inst();
     return G4S_OK;
@@ -762,26 +762,26 @@ int g4s_batch_tune(int accum, const g4s_spec *rep, int32_t trial_steps, int forc
 int g4s_batch_destroy(g4s_batch *b)
 {
     if (!b) return G4S_OK;
-    if (b->stream) hipStreamSynchronize(b->stream);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
     for (hipStream_t st : b->side)
-        if (st) hipStreamSynchronize(st);
+        if (st) (void)hipStreamSynchronize(st);
     drop_graph(b);
     for (Inst &in : b->inst)
-        if (in.block) hipFree(in.block);
+        if (in.block) (void)hipFree(in.block);
     for (void *p : {(void *)b->d_desc, (void *)b->d_work, (void *)b->d_ring, (void *)b->d_bcnt, (void *)b->d_slots,
                     (void *)b->d_counts, (void *)b->d_units, (void *)b->d_ctr, (void *)b->d_done})
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     for (void *p : {(void *)b->h_desc, (void *)b->h_work, (void *)b->h_ring, (void *)b->h_slots, (void *)b->h_counts,
                     (void *)b->h_units, (void *)b->h_ctr})
-        if (p) hipHostFree(p);
-    if (b->ev0) hipEventDestroy(b->ev0);
-    if (b->ev1) hipEventDestroy(b->ev1);
-    if (b->fork) hipEventDestroy(b->fork);
+        if (p) (void)hipHostFree(p);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->fork) (void)hipEventDestroy(b->fork);
     for (int g = 0; g < kMaxGroups - 1; ++g) {
-        if (b->join[g]) hipEventDestroy(b->join[g]);
-        if (b->side[g]) hipStreamDestroy(b->side[g]);
+        if (b->join[g]) (void)hipEventDestroy(b->join[g]);
+        if (b->side[g]) (void)hipStreamDestroy(b->side[g]);
     }
-    if (b->stream) hipStreamDestroy(b->stream);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return G4S_OK;
 }

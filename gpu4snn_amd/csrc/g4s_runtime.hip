@@ -185,11 +185,11 @@ hipError_t kev_collect(g4s_net *net, double *ms)
 int ensure_chunk(g4s_net *net, int chunk)
 {
     if (chunk <= net->chunk_cap) return G4S_OK;
-    if (net->d.raster) { hipFree(net->d.raster); net->allocs.erase(std::find(net->allocs.begin(), net->allocs.end(), (void *)net->d.raster)); }
-    if (net->d.ctl) { hipFree(net->d.ctl); net->allocs.erase(std::find(net->allocs.begin(), net->allocs.end(), (void *)net->d.ctl)); }
+    if (net->d.raster) { (void)hipFree(net->d.raster); net->allocs.erase(std::find(net->allocs.begin(), net->allocs.end(), (void *)net->d.raster)); }
+    if (net->d.ctl) { (void)hipFree(net->d.ctl); net->allocs.erase(std::find(net->allocs.begin(), net->allocs.end(), (void *)net->d.ctl)); }
     for (int k = 0; k < 2; ++k) {
-        if (net->h_raster[k]) hipHostFree(net->h_raster[k]);
-        if (net->h_ctl[k]) hipHostFree(net->h_ctl[k]);
+        if (net->h_raster[k]) (void)hipHostFree(net->h_raster[k]);
+        if (net->h_ctl[k]) (void)hipHostFree(net->h_ctl[k]);
         net->h_raster[k] = nullptr;
         net->h_ctl[k] = nullptr;
     }
@@ -198,7 +198,7 @@ int ensure_chunk(g4s_net *net, int chunk)
     if ((rc = net->alloc(&net->d.raster, (size_t)chunk * net->d.words))) return rc;
     if ((rc = net->alloc(&net->d.ctl, (size_t)chunk * kCtlStride))) return rc;
     if (net->d.bcnt) {
-        hipFree(net->d.bcnt);
+        (void)hipFree(net->d.bcnt);
         net->allocs.erase(std::find(net->allocs.begin(), net->allocs.end(), (void *)net->d.bcnt));
         net->d.bcnt = nullptr;
     }
@@ -288,7 +288,7 @@ int build_slices(g4s_net *net, int want)
                     (long long)P, (long long)L);
     for (void **pp : {(void **)&net->d.soff, (void **)&net->d.bcnt}) {
         if (*pp) {
-            hipFree(*pp);
+            (void)hipFree(*pp);
             net->allocs.erase(std::find(net->allocs.begin(), net->allocs.end(), *pp));
             *pp = nullptr;
         }
@@ -321,8 +321,8 @@ int slice_block(const g4s_net *net, int requested)
 
 void drop_graph(g4s_net *net)
 {
-    if (net->graph_exec) hipGraphExecDestroy(net->graph_exec);
-    if (net->graph) hipGraphDestroy(net->graph);
+    if (net->graph_exec) (void)hipGraphExecDestroy(net->graph_exec);
+    if (net->graph) (void)hipGraphDestroy(net->graph);
     net->graph_exec = nullptr;
     net->graph = nullptr;
     net->graph_steps = 0;
@@ -351,7 +351,7 @@ int run_ab_chunk(g4s_net *net, int cs, bool use_graph, int block, bool det)
         hipGraph_t g = nullptr;
         hipError_t e2 = hipStreamEndCapture(net->stream, &g);
         if (e != hipSuccess || e2 != hipSuccess) {
-            if (g) hipGraphDestroy(g);
+            if (g) (void)hipGraphDestroy(g);
             return fail(G4S_ERR_HIP, "graph capture of %d steps: %s", cs, hipGetErrorString(e != hipSuccess ? e : e2));
         }
         net->graph = g;
@@ -642,7 +642,7 @@ int run_impl(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, 
     };
     // drain what is still in flight when the run stops early
     auto abandon = [&](int code) {
-        for (int k = 0; k < npend; ++k) hipEventSynchronize(net->done_ev[pend[k].buf]);
+        for (int k = 0; k < npend; ++k) (void)hipEventSynchronize(net->done_ev[pend[k].buf]);
         return code;
     };
 
@@ -802,22 +802,22 @@ int g4s_get_synapses(g4s_net *net, uint32_t *post, float *w)
 int g4s_destroy(g4s_net *net)
 {
     if (!net) return G4S_OK;
-    if (net->stream) hipStreamSynchronize(net->stream);
+    if (net->stream) (void)hipStreamSynchronize(net->stream);
     drop_graph(net);
-    for (void *p : net->allocs) hipFree(p);
-    if (net->h_tinfo) hipHostFree(net->h_tinfo);
-    if (net->h_err) hipHostFree(net->h_err);
+    for (void *p : net->allocs) (void)hipFree(p);
+    if (net->h_tinfo) (void)hipHostFree(net->h_tinfo);
+    if (net->h_err) (void)hipHostFree(net->h_err);
     for (int k = 0; k < 2; ++k) {
-        if (net->h_raster[k]) hipHostFree(net->h_raster[k]);
-        if (net->h_ctl[k]) hipHostFree(net->h_ctl[k]);
+        if (net->h_raster[k]) (void)hipHostFree(net->h_raster[k]);
+        if (net->h_ctl[k]) (void)hipHostFree(net->h_ctl[k]);
     }
-    for (hipEvent_t e : net->kev) hipEventDestroy(e);
+    for (hipEvent_t e : net->kev) (void)hipEventDestroy(e);
     for (int k = 0; k < 2; ++k) {
-        if (net->ev0[k]) hipEventDestroy(net->ev0[k]);
-        if (net->ev1[k]) hipEventDestroy(net->ev1[k]);
-        if (net->done_ev[k]) hipEventDestroy(net->done_ev[k]);
+        if (net->ev0[k]) (void)hipEventDestroy(net->ev0[k]);
+        if (net->ev1[k]) (void)hipEventDestroy(net->ev1[k]);
+        if (net->done_ev[k]) (void)hipEventDestroy(net->done_ev[k]);
     }
-    if (net->stream) hipStreamDestroy(net->stream);
+    if (net->stream) (void)hipStreamDestroy(net->stream);
     delete net;
     return G4S_OK;
 }
@@ -838,7 +838,7 @@ int g4s_build_synapses_device(const g4s_spec *spec, uint32_t *post, float *w)
         if (hipMemcpy(tab.data(), d, ns * sizeof(uint2), hipMemcpyDeviceToHost) != hipSuccess)
             rc = fail(G4S_ERR_HIP, "copy of the device table failed");
     }
-    hipFree(d);
+    (void)hipFree(d);
     if (rc) return rc;
     for (size_t k = 0; k < ns; ++k) {
         post[k] = tab[k].x;
@@ -854,7 +854,7 @@ int g4s_device_thalamic(uint64_t seed, uint32_t t, int64_t n, float *out)
     HIP_OK(hipMalloc((void **)&dout, (size_t)n * sizeof(float)));
     hipError_t e = launch_thalamic(seed, t, n, dout, nullptr);
     if (e == hipSuccess) e = hipMemcpy(out, dout, (size_t)n * sizeof(float), hipMemcpyDeviceToHost);
-    hipFree(dout);
+    (void)hipFree(dout);
     if (e != hipSuccess) return fail(G4S_ERR_HIP, "device thalamic: %s", hipGetErrorString(e));
     return G4S_OK;
 }
@@ -865,16 +865,15 @@ int g4s_device_normal_words(const uint32_t *x0, const uint32_t *x1, int64_t n, f
     uint32_t *d0 = nullptr, *d1 = nullptr;
     float *dout = nullptr;
     const size_t b = (size_t)n * 4;
-    HIP_OK(hipMalloc((void **)&d0, b));
-    HIP_OK(hipMalloc((void **)&d1, b));
-    HIP_OK(hipMalloc((void **)&dout, b));
-    hipError_t e = hipMemcpy(d0, x0, b, hipMemcpyHostToDevice);
+    hipError_t e = hipMalloc((void **)&d0, b);
+    if (e == hipSuccess) e = hipMalloc((void **)&d1, b);
+    if (e == hipSuccess) e = hipMalloc((void **)&dout, b);
+    if (e == hipSuccess) e = hipMemcpy(d0, x0, b, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d1, x1, b, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = launch_normal_words(d0, d1, n, dout, nullptr);
     if (e == hipSuccess) e = hipMemcpy(out, dout, b, hipMemcpyDeviceToHost);
-    hipFree(d0);
-    hipFree(d1);
-    hipFree(dout);
+    for (void *p : {(void *)d0, (void *)d1, (void *)dout})
+        if (p) (void)hipFree(p);
     if (e != hipSuccess) return fail(G4S_ERR_HIP, "device normals: %s", hipGetErrorString(e));
     return G4S_OK;
 }
