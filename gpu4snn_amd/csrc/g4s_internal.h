@@ -60,7 +60,7 @@ struct DevNet {
     uint32_t *bar;
     int32_t *err;
     // Slice partition of the targets for the LDS-accumulating schemes: slice
-    // p owns neurons [p*L, min((p+1)*L, n)), L a multiple of 64.  soff[i*(P+1)
+    // p owns neurons [p*L, min((p+1)*L, n)), L a multiple of 32.  soff[i*(P+1)
     // + p] is the index in row i of its first target >= p*L (rows ascending).
     const uint16_t *soff;
     int32_t P, L;

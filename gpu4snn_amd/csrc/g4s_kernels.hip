@@ -77,7 +77,7 @@ __device__ __forceinline__ void emit(const DevNet &d, int t, int rel, int wave_b
     const int lane = lane_id();
     uint32_t *bm = (t & 1) ? d.bm1 : d.bm0;
     const int w = (wave_base >> 5) + lane;
-    if (lane < 2 && w < d.words) {
+    if (lane < 2 && w < d.words && (w << 5) < limit) {
         const uint32_t bits = lane ? (uint32_t)(mask >> 32) : (uint32_t)mask;
         bm[w] = bits;
         if (d.raster) d.raster[(size_t)rel * (size_t)d.words + (size_t)w] = bits;
@@ -456,11 +456,14 @@ __global__ void __launch_bounds__(256) k_ap(DevNet d, int t, int rel, int last, 
 // pending-input array in HBM; the owner then updates its neurons directly.
 // ---------------------------------------------------------------------------
 
-// Workgroups are dealt to the 8 XCDs round-robin; consecutive slices go to
-// the same XCD so rows' neighbouring segments and offsets share its L2.
-__device__ __forceinline__ int slice_of_block(int b, int g)
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup b to XCD b mod
+// 8); consecutive slices go to the same XCD so rows' neighbouring segments
+// and offsets share its L2.  XCD x holds g/8 workgroups, one more if
+// x < g mod 8, and takes the next run of slices: a bijection for any g.
+__host__ __device__ inline int slice_of_block(int b, int g)
 {
-    return (g % 8 == 0) ? (b % 8) * (g / 8) + b / 8 : b;
+    const int x = b % 8, q = g / 8, r = g % 8;
+    return x * q + (x < r ? x : r) + b / 8;
 }
 
 // Bitmap word w; kCoh: an agent-scope atomic load.
@@ -474,9 +477,11 @@ __device__ __forceinline__ uint32_t bm_word(P bm, int w)
 }
 
 // Bitmap and raster words of one wave covering [wave_base, wave_base+64)
-// of a slice kernel; returns the wave's spike mask.  Waves starting at or
-// beyond `limit` write nothing.  kCoh: bitmap words are stored with
-// agent-scope atomics (read by other workgroups of the same launch).
+// of a slice kernel; returns the wave's spike mask.  A word is written only
+// if it starts below `limit` (the end of the slice): slices are multiples of
+// 32 neurons, so the second word of a slice's last wave may be the next
+// slice's.  kCoh: bitmap words are stored with agent-scope atomics (read by
+// other workgroups of the same launch).
 template <bool kCoh = false, class Net>
 __device__ __forceinline__ uint64_t emit_bits(const Net &d, int t, int rel, int wave_base, bool spk, int limit)
 {
@@ -485,7 +490,7 @@ __device__ __forceinline__ uint64_t emit_bits(const Net &d, int t, int rel, int 
     const int lane = lane_id();
     auto bm = (t & 1) ? d.bm1 : d.bm0;
     const int w = (wave_base >> 5) + lane;
-    if (lane < 2 && w < d.words) {
+    if (lane < 2 && w < d.words && (w << 5) < limit) {
         const uint32_t bits = lane ? (uint32_t)(mask >> 32) : (uint32_t)mask;
         if (kCoh)
             __hip_atomic_store(&bm[w], bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

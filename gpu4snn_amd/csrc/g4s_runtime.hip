@@ -262,13 +262,13 @@ int coop_launch(g4s_net *net, CoopKernel k, int block, int64_t useful_threads, L
 }
 
 // Slice partition for the LDS-accumulating schemes: `want` slices (0 = one
-// per CU), slice length L a multiple of 64 and at most 16*1024 neurons;
-// offsets stored as uint16 (Nsyn < 65536).  Rebuilt when the slice count
-// changes.
+// per CU), slice length L a multiple of 32 (whole bitmap words) and at most
+// 16*1024 neurons; offsets stored as uint16 (Nsyn < 65536).  Rebuilt when the
+// slice count changes.
 int64_t pick_slice_len(int64_t n, int64_t want)
 {
     int64_t P = std::max<int64_t>(1, std::min<int64_t>(want, (n + 63) / 64));
-    int64_t L = ((n + P - 1) / P + 63) / 64 * 64;
+    int64_t L = ((n + P - 1) / P + 31) / 32 * 32;
     return std::min<int64_t>(L, 16 * 1024);
 }
 
@@ -316,7 +316,8 @@ int build_slices(g4s_net *net, int want)
 int slice_block(const g4s_net *net, int requested)
 {
     if (requested > 0) return requested;
-    return (int)std::min<int64_t>(1024, std::max<int64_t>(64, net->d.L));
+    // one thread per slice neuron, in whole waves
+    return (int)std::min<int64_t>(1024, std::max<int64_t>(64, ((int64_t)net->d.L + 63) / 64 * 64));
 }
 
 void drop_graph(g4s_net *net)

@@ -39,7 +39,7 @@ def f64_oracle(oracle, big_net):
 @pytest.mark.parametrize("alg", ["AB", "SKL"])
 def test_cfg2_fp32_voltage_error(g4s, big_net, f64_oracle, alg):
     big_net.reset()
-    # 256 slices of 3968 neurons: SKL needs every slice co-resident
+    # 256 slices requested (255 of 3936 neurons): SKL needs every slice co-resident
     words, counts = big_net.run(STEPS, alg=alg, accum="atomic", slices=256, block=1024)
     v, _, _, _ = big_net.state()
     rate = counts.sum() / N / (STEPS / 1000.0)

@@ -154,7 +154,7 @@ def test_cfg1_bench_plan_det_bitwise(g4s, big, big_oracle, alg):
     onet, oraster, _ = big_oracle
     with g4s.Network(n, nsyn, seed) as net:
         words, _ = net.run(60, alg=alg, accum="det", slices=256, block=1024)
-        assert net.timing()["grid"] == 224
+        assert net.timing()["grid"] == 241  # slices of 416 neurons
         assert np.array_equal(dense(g4s, words, n), oraster)
         v, u, _, _ = net.state()
         assert np.array_equal(v.view(np.uint32), onet.v.view(np.uint32))
@@ -225,8 +225,29 @@ def test_default_tiling_fits_det_workspace(g4s, oracle):
         net.reset()
         net.run(steps, alg="AB", accum="atomic")
         p_atomic = net.timing()["grid"]
-    # lengths round up to multiples of 64: 509 slices of 7872 vs 256 of 15680
+    # lengths round up to multiples of 32: 511 slices of 7840 vs 256 of 15648
     assert p_det > 3 * p_atomic // 2 > 0, (p_det, p_atomic)
+
+
+@pytest.mark.parametrize("block", [64, 256])
+def test_slices_of_odd_word_pairs(g4s, oracle, block):
+    """Slice lengths are multiples of 32 neurons (whole bitmap words), not of
+    64: at N=3000 with 11 slices requested each slice holds 288 neurons, so
+    the last wave of a slice covers 32 neurons of the next one and must leave
+    that slice's bitmap word alone.  Det is bit-identical to the oracle; the
+    atomic run keeps the oracle's raster for its first 100 steps."""
+    n, nsyn, seed, steps = 3000, 300, 41, 300
+    onet, oraster, _ = oracle_raster(oracle, n, nsyn, steps, seed, regime=2)
+    for alg in ("AB", "SKL"):
+        with g4s.Network(n, nsyn, seed, regime=2) as net:
+            words, _ = net.run(steps, alg=alg, accum="det", slices=11, block=block)
+            assert net.timing()["grid"] == 11
+            assert np.array_equal(dense(g4s, words, n), oraster), (alg, block)
+            v, _, _, _ = net.state()
+            assert np.array_equal(v.view(np.uint32), onet.v.view(np.uint32)), (alg, block)
+            net.reset()
+            words, _ = net.run(100, alg=alg, accum="atomic", slices=11, block=block)
+            assert np.array_equal(dense(g4s, words, n), oraster[:100]), (alg, block)
 
 
 @pytest.mark.parametrize("block", [256, 1024])
