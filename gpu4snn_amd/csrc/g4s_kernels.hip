@@ -554,13 +554,13 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int *s_tmp, int *tota
 // Large single-network slices (kLargeSlice neurons or more: networks of
 // about 10^6 neurons) take more spikes per pass and twice the entry loads in
 // flight, cutting the dependent levels of a step whose spikes need several
-// passes: 8 spikes per thread in atomic mode, 4 in det mode (whose LDS
-// also holds the window's counting sort).  Det finds each entry's spike by
-// the owner map (measured 2.5% faster at N=10^6 than the search; the same in
-// atomic mode).
+// passes: 8 spikes per thread in both modes (det's workspace, which also
+// holds the window's counting sort, then fits slices up to about 3970
+// neurons at 1024 threads).  Det finds each entry's spike by the owner map
+// (measured 2.5% faster at N=10^6 than the search; the same in atomic mode).
 constexpr int kSpikesSingle = 2, kEntriesSingle = 4;
 constexpr int kLargeSlice = 2048, kEntriesLarge = 8;
-constexpr int kSpikesLargeAtomic = 8, kSpikesLargeDet = 4;
+constexpr int kSpikesLargeAtomic = 8, kSpikesLargeDet = 8;
 constexpr bool kOwnerLargeAtomic = false, kOwnerLargeDet = true;
 constexpr int kEntriesBatch = 8;
 // batched det slices keep 2 spikes per thread, which keeps their LDS

@@ -54,3 +54,23 @@ def test_cfg2_fp32_voltage_error(g4s, big_net, f64_oracle, alg):
     assert 5.0 < rate < 60.0
     assert same.mean() >= 0.80
     assert p99 < 1.0
+
+
+@pytest.mark.parametrize("alg", ["AB", "SKL"])
+def test_cfg2_det_bitwise(g4s, oracle, big_net, alg):
+    """Det mode at N=10^6 (255 slices of 3936 neurons, 1024 threads: the large
+    plan, about 16k spikes per step in two passes) is bit-identical to the fp32
+    oracle run on the same synapse table for the first 20 steps."""
+    steps = 20
+    post, w = big_net.synapses()
+    onet = oracle.Network(N, NSYN, SEED, regime=REGIME, table=(post, w))
+    oraster, ocounts, _ = onet.run(steps, threads=16)
+    del post, w
+    big_net.reset()
+    words, counts = big_net.run(steps, alg=alg, accum="det", slices=256, block=1024)
+    assert big_net.timing()["grid"] == 255
+    assert np.array_equal(counts, ocounts), alg
+    assert np.array_equal(g4s.raster_to_dense(words, N), oraster), alg
+    v, u, _, _ = big_net.state()
+    assert np.array_equal(v.view(np.uint32), onet.v.view(np.uint32)), alg
+    assert np.array_equal(u.view(np.uint32), onet.u.view(np.uint32)), alg
