@@ -244,12 +244,21 @@ __device__ __forceinline__ void push_flat(const DevNet &d, int t, int count, int
 // before the workgroup arrives, and the loads after the barrier are issued
 // only once the final arrival has been seen.  It pays no release (an L2
 // write-back of the XCD) and no acquire (an L2 invalidate, which would also
-// evict the static synapse rows that the next step reads).
-__device__ bool grid_sync(const DevNet &d, uint32_t &epoch)
+// evict the static synapse rows that the next step reads).  Between its two
+// workgroup barriers a non-polling wave stores the step's spike count.
+__device__ bool grid_sync(const DevNet &d, uint32_t &epoch, const int *s_wcnt, int32_t *cnt_dst)
 {
     __shared__ int s_ok;
     __builtin_amdgcn_s_waitcnt(0);  // this wave's stores have completed
     __syncthreads();
+    // the workgroup's spike count of the step (s_wcnt: one total per wave,
+    // written before the call), summed by a wave that does not poll
+    const int nw = (int)blockDim.x / kWave;
+    if ((int)threadIdx.x == (nw > 1 ? kWave : 0)) {
+        int c = 0;
+        for (int w = 0; w < nw; ++w) c += s_wcnt[w];
+        *cnt_dst = c;
+    }
     epoch += 1;
     if (threadIdx.x < kWave) {
         const int lane = (int)threadIdx.x;
@@ -1272,7 +1281,7 @@ template <int R, bool kDet, bool kLarge>
 __global__ void __launch_bounds__(1024) k_skl_slice(DevNet d, int t0, int steps, int last_t)
 {
     extern __shared__ __align__(16) float dyn[];  // slice_accumulate needs 16-byte alignment
-    __shared__ int s_cnt;
+    __shared__ int s_wcnt[1024 / kWave];  // per-wave spike totals of the step
     float *acc = dyn;
     const int p = slice_of_block((int)blockIdx.x, (int)gridDim.x);
     const int lo = p * d.L;
@@ -1295,7 +1304,6 @@ __global__ void __launch_bounds__(1024) k_skl_slice(DevNet d, int t0, int steps,
         }
     }
     uint32_t epoch = 0;
-    if (tid == 0) s_cnt = 0;  // slice_accumulate's barriers precede slice_count
     for (int rel = 0; rel < steps; ++rel) {
         const int t = t0 + rel;
         using Plan = SlicePlan<kDet, kLarge>;
@@ -1318,8 +1326,12 @@ __global__ void __launch_bounds__(1024) k_skl_slice(DevNet d, int t0, int steps,
             }
             masks[k] = emit_bits<true>(d, t, rel, lo + k * B + (tid & ~(kWave - 1)), spk, lo + len);
         }
-        slice_count<R>(d, rel, p, ntile, masks, &s_cnt);
-        if (!grid_sync(d, epoch)) break;
+        int c = 0;
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+            if (k < ntile) c += __popcll(masks[k]);
+        if (lane_id() == 0) s_wcnt[tid / kWave] = c;
+        if (!grid_sync(d, epoch, s_wcnt, &d.bcnt[(size_t)rel * (size_t)d.P + (size_t)p])) break;
     }
 #pragma unroll
     for (int k = 0; k < R; ++k) {
