@@ -161,7 +161,7 @@ def bench_batch(g, args):
         "config": {"workload": f"{len(specs)} networks, N log-uniform [1e3, 5e4], Nsyn in {{100,500,1000}}, "
                                f"thalamic x U(0.8,1.5), {steps_timed} timed steps each",
                    "accum": args.accum, "chunk": args.chunk,
-                   "slice_len": args.slice_len or ("tuned per bucket" if tuned else "min(512, N)"),
+                   "slice_len": args.slice_len or ("tuned per bucket" if tuned else "min(768, N)"),
                    "tuned": {f"log2N={k[0]},Nsyn={k[1]}": v for k, v in sorted(tuned.items())} or None,
                    "schedule": args.schedule, "block": args.batch_block or 256, "streams": args.streams or 3,
                    "neurons": sum(x[0] for x in specs),
@@ -171,7 +171,7 @@ def bench_batch(g, args):
         "speedup_vs_plain": (sim_s / wall) / (p_sim / p_wall),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic_of(f"k_batch_step|batch{len(specs)}|{args.accum}|chunk={args.chunk}|slice_len=512"
+                     "traffic": traffic_of(f"k_batch_step|batch{len(specs)}|{args.accum}|chunk={args.chunk}|slice_len=768"
                                            f"|streams={args.streams or 3}|per_step")
                      if not args.slice_len and not args.tune and args.schedule == "step" and args.seed == 20221 else None,
                      "kernel": "k_batch_step",

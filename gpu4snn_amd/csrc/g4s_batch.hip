@@ -59,14 +59,19 @@ Layout layout_of(int64_t n, int32_t nsyn, int64_t P)
     return l;
 }
 
-// Default slice length: 512 neurons (fewer when N is smaller).  Measured on
-// 256 cfg3-like networks: 512 beats 128, 256 and 1024, and a Nsyn-scaled
-// choice (about one 2048-entry window per slice and step) is slower -- the
-// fixed part of a workgroup (bitmap scan, offsets, update) dominates.
+// Default slice length: 768 neurons (fewer when N is smaller).  Measured on
+// 256 networks with N log-uniform on [1e3, 5e4] and Nsyn in {100, 500,
+// 1000}, 256-thread workgroups: atomic mode 3205-3212 simulated seconds per
+// second at 768, against 2396 (384), 2788 (512), 3027 (640), 3045 (704),
+// 3094 (832), 3131 (896) and 3117 (1024); det mode 2025 at 768, 2011 at 512,
+// 1962 at 640, 1752 at 1024.  Whole multiples of the workgroup (3 neurons per
+// thread) win; fewer, longer slices pay the fixed part of a workgroup (bitmap
+// scan, offsets, update) less often.  A Nsyn-scaled choice (about one
+// 2048-entry window per slice and step) is slower.
 int default_slice_len(int64_t n, int32_t nsyn)
 {
     (void)nsyn;
-    return (int)std::min<int64_t>(512, (n + 63) / 64 * 64);
+    return (int)std::min<int64_t>(768, (n + 63) / 64 * 64);
 }
 
 struct Inst {
@@ -704,7 +709,7 @@ int g4s_batch_tune(int accum, const g4s_spec *rep, int32_t trial_steps, int forc
     const int warm = std::max(20, trial_steps / 4);
     TunedLen best{0, INFINITY, 0.0f};
     int copies = 0;
-    for (int L : {128, 256, 512, 1024, 2048}) {
+    for (int L : {128, 256, 512, 768, 1024, 2048}) {
         if (L > (n + 63) / 64 * 64 * 2 && L > 128) continue;  // one slice already covers N
         if (batch_lds(L, accum == G4S_ACCUM_DET) > 64 * 1024) continue;
         const int64_t P = (n + L - 1) / L;

@@ -573,7 +573,7 @@ constexpr int kSpikesLargeAtomic = 8, kSpikesLargeDet = 8;
 constexpr bool kOwnerLargeAtomic = false, kOwnerLargeDet = true;
 constexpr int kEntriesBatch = 8;
 // batched det slices keep 2 spikes per thread, which keeps their LDS
-// within 7 workgroups per CU at 512-neuron slices
+// within 7 workgroups per CU at 512-neuron slices (6 at the default 768)
 template <bool kDet>
 constexpr int spikes_batch() { return kDet ? 2 : 4; }
 // How an entry of a window finds its spike: an owner map (batched slices,
@@ -1171,8 +1171,9 @@ __device__ __forceinline__ void batch_unit(const BNet &d, int rel, int p, float 
 // One launch per step: workgroup b takes work unit work[b] = {slot, p} at
 // step rel; the host orders the units so that each XCD holds whole
 // instances.  Instances whose chunk ends before rel do nothing.
-// 256-thread workgroups: 7 waves per SIMD, the most that keep every value
-// in registers (and whose LDS fits a CU at 512-neuron slices).
+// 256-thread workgroups: 7 waves per SIMD (72 VGPRs, 8 bytes of scratch per
+// thread; more waves spill more), and 7 workgroups' LDS fits a CU at the
+// default 768-neuron slices in atomic mode.
 template <bool kDet, int kB>
 __global__ void __launch_bounds__(kB, kB == 256 ? 7 : 1) k_batch_step(const DevNet *__restrict__ descs,
                                                                      const uint2 *__restrict__ work, int rel)

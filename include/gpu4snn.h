@@ -194,7 +194,7 @@ typedef struct g4s_batch g4s_batch;
 typedef struct g4s_batch_opts {
     int32_t chunk;       /* steps per chunk, 0 = 100                                */
     int32_t slice_len;   /* neurons per slice (multiple of 64), 0 = the bucket's
-                            g4s_batch_tune choice, else min(512, N)              */
+                            g4s_batch_tune choice, else min(768, N)              */
     int32_t graph;       /* schedule 0: >= 0 replays the chunk's launches from a HIP
                             graph, < 0 launches them directly                       */
     int32_t schedule;    /* 0: one launch per step over all instances;
