@@ -196,7 +196,7 @@ def bench_serve(g, args):
         t_arr.append(t)
     specs = batch_specs(len(t_arr), args.seed + 2)
     steps = [int(x) for x in rng.integers(100, 5001, len(t_arr))]
-    srv = g.Server(args.accum, chunk=args.chunk, budget=int(args.budget_gb * (1 << 30)))
+    srv = g.Server(args.accum, chunk=args.chunk, budget=int(args.budget_gb * (1 << 30)), slice_len=args.slice_len)
     ids = []
     t0 = time.perf_counter()
     last_report = t0
