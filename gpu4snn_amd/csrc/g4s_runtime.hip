@@ -937,7 +937,7 @@ extern "C" int g4s_autotune(g4s_net *net, int accum, int32_t trial_steps, int fo
     for (int alg = G4S_ALG_AP; alg <= G4S_ALG_SKL; ++alg) {
         const bool slice = alg == G4S_ALG_AB || alg == G4S_ALG_SKL;
         if (slice) {
-            for (int P : {64, 128, 256, 512})
+            for (int P : {64, 128, 192, 256, 512})
                 for (int B : {256, 512, 1024}) {
                     g4s_plan c{};
                     c.alg = alg; c.accum = accum; c.opts.slices = P; c.opts.block = B;
