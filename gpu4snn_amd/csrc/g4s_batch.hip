@@ -32,6 +32,15 @@ namespace {
 constexpr int kBlock = 256;          // threads per workgroup of the batched kernels
 constexpr int kDefaultChunk = 100;
 constexpr int kMaxGroups = 4;         // schedule 0: instance groups on their own streams
+// Default group count.  At 768-neuron slices, 256 networks, in a process
+// whose only streams are the batch's: 4 groups run 3237-3249 (atomic) and
+// 2089-2092 (det) simulated seconds per second, 3 groups 3182-3206 and
+// 2012-2021, 2 groups 2942 and 1811 (one box).  With one more stream in the
+// process (a single network's, as in the benchmark that runs both), 4 groups
+// fell to 2890 while 3 groups kept 3172-3227, most likely because the
+// process has 4 hardware queues and a fifth stream then shares one with a
+// group.  3 is the default.
+constexpr int kDefaultGroups = 3;
 constexpr int kMaxSliceLen = 4096;
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
@@ -257,7 +266,7 @@ int g4s_batch_create(int accum, const g4s_batch_opts *opts, g4s_batch **out)
     b->opts = o;
     b->chunk = o.chunk > 0 ? o.chunk : kDefaultChunk;
     b->block = o.block > 0 ? o.block : kBlock;
-    b->groups = o.schedule == 1 ? 1 : (o.streams > 0 ? o.streams : 3);
+    b->groups = o.schedule == 1 ? 1 : (o.streams > 0 ? o.streams : kDefaultGroups);
     bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
               hipEventCreate(&b->ev0) == hipSuccess && hipEventCreate(&b->ev1) == hipSuccess;
     if (ok && b->groups > 1) ok = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess;
