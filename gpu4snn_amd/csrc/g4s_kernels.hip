@@ -571,9 +571,13 @@ constexpr int kSpikesSingle = 2, kEntriesSingle = 4;
 constexpr int kLargeSlice = 2048, kEntriesLarge = 8;
 constexpr int kSpikesLargeAtomic = 8, kSpikesLargeDet = 8;
 constexpr bool kOwnerLargeAtomic = false, kOwnerLargeDet = true;
-constexpr int kEntriesBatch = 8;
-// batched det slices keep 2 spikes per thread, which keeps their LDS
-// within 7 workgroups per CU at 512-neuron slices (6 at the default 768)
+// Batched slices: 8 entry loads in flight per thread in atomic mode, 6 in
+// det mode, and 2 spikes per pass per thread in det mode: det's workspace
+// (its window's counting sort) then fits 7 workgroups per CU at the default
+// 768-neuron slices (22.6 KB; 25.6 KB and 6 workgroups with 8 entries,
+// measured 1.8% slower).
+template <bool kDet>
+constexpr int entries_batch() { return kDet ? 6 : 8; }
 template <bool kDet>
 constexpr int spikes_batch() { return kDet ? 2 : 4; }
 // How an entry of a window finds its spike: an owner map (batched slices,
@@ -1151,7 +1155,7 @@ __device__ __forceinline__ void batch_unit(const BNet &d, int rel, int p, float 
     float *acc = dyn;
     const int B = (int)blockDim.x;
     const int tid = (int)threadIdx.x;
-    slice_accumulate<kDet, kEntriesBatch, spikes_batch<kDet>(), kOwnerBatch>(d, t, p, lo, len, dyn);
+    slice_accumulate<kDet, entries_batch<kDet>(), spikes_batch<kDet>(), kOwnerBatch>(d, t, p, lo, len, dyn);
     const bool last = t == d.last_t;
     if (tid == 0) *s_cnt = 0;
     __syncthreads();
@@ -1171,9 +1175,9 @@ __device__ __forceinline__ void batch_unit(const BNet &d, int rel, int p, float 
 // One launch per step: workgroup b takes work unit work[b] = {slot, p} at
 // step rel; the host orders the units so that each XCD holds whole
 // instances.  Instances whose chunk ends before rel do nothing.
-// 256-thread workgroups: 7 waves per SIMD (72 VGPRs, 8 bytes of scratch per
-// thread; more waves spill more), and 7 workgroups' LDS fits a CU at the
-// default 768-neuron slices in atomic mode.
+// 256-thread workgroups: 7 waves per SIMD (72 VGPRs and 8 bytes of scratch
+// per thread in atomic mode, 66 VGPRs in det mode; more waves spill more),
+// and 7 workgroups' LDS fits a CU at the default 768-neuron slices.
 template <bool kDet, int kB>
 __global__ void __launch_bounds__(kB, kB == 256 ? 7 : 1) k_batch_step(const DevNet *__restrict__ descs,
                                                                      const uint2 *__restrict__ work, int rel)
@@ -1758,7 +1762,8 @@ int skl_slice_blocks_per_cu(int L, int block, bool det)
 size_t slice_lds(int L, int block, bool det) { return single_slice_lds(L, block, det); }
 size_t batch_lds(int L, bool det, int block)
 {
-    return slice_lds_bytes(L, block, det, kEntriesBatch, det ? spikes_batch<true>() : spikes_batch<false>(), kOwnerBatch);
+    return slice_lds_bytes(L, block, det, det ? entries_batch<true>() : entries_batch<false>(),
+                           det ? spikes_batch<true>() : spikes_batch<false>(), kOwnerBatch);
 }
 
 hipError_t launch_pull_thread(const DevNet &d, int t, int rel, int last, const Launch &L)

@@ -83,16 +83,19 @@ def test_batch_det_slice_lengths(g4s, oracle, slice_len):
 
 
 def test_batch_slice_length_beyond_lds(g4s):
-    """det needs more LDS per slice than atomic: 4096-neuron slices are
-    refused in det mode (G4S_ERR_UNSUPPORTED) and run in atomic mode."""
-    with g4s.Batch("det", slice_len=4096) as b:
+    """det needs more LDS per slice than atomic: 4096-neuron slices at
+    1024 threads (about 100 KB of det workspace) are refused in det mode
+    (G4S_ERR_UNSUPPORTED, a batched workgroup may use 64 KB); at 256 threads
+    they run in both modes (62.5 KB det, 34 KB atomic)."""
+    with g4s.Batch("det", slice_len=4096, block=1024) as b:
         with pytest.raises(g4s.G4SError) as e:
             b.add(5000, 100, 1, 10)
         assert e.value.code == -5
-    with g4s.Batch("atomic", slice_len=4096) as b:
-        s = b.add(5000, 100, 1, 10)
-        b.run()
-        assert b.info(s)["done"] == 1 and b.info(s)["slices"] == 2
+    for accum, block in (("det", 256), ("atomic", 256)):
+        with g4s.Batch(accum, slice_len=4096, block=block) as b:
+            s = b.add(5000, 100, 1, 10)
+            b.run()
+            assert b.info(s)["done"] == 1 and b.info(s)["slices"] == 2, (accum, block)
 
 
 def test_batch_continuous_join_and_slot_reuse(g4s, oracle):
