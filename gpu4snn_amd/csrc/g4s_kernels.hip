@@ -571,15 +571,17 @@ constexpr int kSpikesSingle = 2, kEntriesSingle = 4;
 constexpr int kLargeSlice = 2048, kEntriesLarge = 8;
 constexpr int kSpikesLargeAtomic = 8, kSpikesLargeDet = 8;
 constexpr bool kOwnerLargeAtomic = false, kOwnerLargeDet = true;
-// Batched slices: 8 entry loads in flight per thread in atomic mode, 6 in
-// det mode, and 2 spikes per pass per thread in det mode: det's workspace
-// (its window's counting sort) then fits 7 workgroups per CU at the default
-// 768-neuron slices (22.6 KB; 25.6 KB and 6 workgroups with 8 entries,
-// measured 1.8% slower).
+// Batched slices: 6 entry loads in flight per thread, and 6 spikes per pass
+// per thread in atomic mode, 2 in det mode.  At the default 768-neuron
+// slices 7 workgroups' LDS then fits a CU in both modes (atomic 21.3 KB,
+// det 22.6 KB, whose window also holds its counting sort).  Measured on 256
+// networks: atomic 3282-3294 simulated seconds per second against 3200-3219
+// with 8 entries and 4 spikes (the register savings remove its spill); det
+// 2051-2055 against 2016-2022 with 8 entries (25.6 KB, 6 workgroups).
 template <bool kDet>
-constexpr int entries_batch() { return kDet ? 6 : 8; }
+constexpr int entries_batch() { return 6; }
 template <bool kDet>
-constexpr int spikes_batch() { return kDet ? 2 : 4; }
+constexpr int spikes_batch() { return kDet ? 2 : 6; }
 // How an entry of a window finds its spike: an owner map (batched slices,
 // whose workgroups are issue-bound: a few instructions per entry) or a
 // binary search over the segment offsets, all of a thread's entries in
@@ -1175,9 +1177,8 @@ __device__ __forceinline__ void batch_unit(const BNet &d, int rel, int p, float 
 // One launch per step: workgroup b takes work unit work[b] = {slot, p} at
 // step rel; the host orders the units so that each XCD holds whole
 // instances.  Instances whose chunk ends before rel do nothing.
-// 256-thread workgroups: 7 waves per SIMD (72 VGPRs and 8 bytes of scratch
-// per thread in atomic mode, 66 VGPRs in det mode; more waves spill more),
-// and 7 workgroups' LDS fits a CU at the default 768-neuron slices.
+// 256-thread workgroups: 7 waves per SIMD (66 VGPRs, no scratch, in both
+// modes), and 7 workgroups' LDS fits a CU at the default 768-neuron slices.
 template <bool kDet, int kB>
 __global__ void __launch_bounds__(kB, kB == 256 ? 7 : 1) k_batch_step(const DevNet *__restrict__ descs,
                                                                      const uint2 *__restrict__ work, int rel)
