@@ -229,24 +229,25 @@ def test_default_tiling_fits_det_workspace(g4s, oracle):
     assert p_det > 3 * p_atomic // 2 > 0, (p_det, p_atomic)
 
 
-@pytest.mark.parametrize("block", [64, 256])
-def test_slices_of_odd_word_pairs(g4s, oracle, block):
+@pytest.mark.parametrize("block,slices", [(64, 11), (256, 11), (64, 32)])
+def test_slices_of_odd_word_pairs(g4s, oracle, block, slices):
     """Slice lengths are multiples of 32 neurons (whole bitmap words), not of
     64: at N=3000 with 11 slices requested each slice holds 288 neurons, so
     the last wave of a slice covers 32 neurons of the next one and must leave
-    that slice's bitmap word alone.  Det is bit-identical to the oracle; the
+    that slice's bitmap word alone; with 32 slices each holds 96 neurons, and
+    at 64 threads every slice's second wave spans two slices.  Det is bit-identical to the oracle; the
     atomic run keeps the oracle's raster for its first 100 steps."""
     n, nsyn, seed, steps = 3000, 300, 41, 300
     onet, oraster, _ = oracle_raster(oracle, n, nsyn, steps, seed, regime=2)
     for alg in ("AB", "SKL"):
         with g4s.Network(n, nsyn, seed, regime=2) as net:
-            words, _ = net.run(steps, alg=alg, accum="det", slices=11, block=block)
-            assert net.timing()["grid"] == 11
+            words, _ = net.run(steps, alg=alg, accum="det", slices=slices, block=block)
+            assert net.timing()["grid"] == slices
             assert np.array_equal(dense(g4s, words, n), oraster), (alg, block)
             v, _, _, _ = net.state()
             assert np.array_equal(v.view(np.uint32), onet.v.view(np.uint32)), (alg, block)
             net.reset()
-            words, _ = net.run(100, alg=alg, accum="atomic", slices=11, block=block)
+            words, _ = net.run(100, alg=alg, accum="atomic", slices=slices, block=block)
             assert np.array_equal(dense(g4s, words, n), oraster[:100]), (alg, block)
 
 
