@@ -1167,10 +1167,9 @@ __device__ __forceinline__ void batch_unit(const BNet &d, int rel, int p, float 
     float *acc = dyn;
     const int B = (int)blockDim.x;
     const int tid = (int)threadIdx.x;
+    if (tid == 0) *s_cnt = 0;  // slice_accumulate's barriers precede the count atomics
     slice_accumulate<kDet, entries_batch<kDet>(), spikes_batch<kDet>(), kOwnerBatch>(d, t, p, lo, len, dyn);
     const bool last = t == d.last_t;
-    if (tid == 0) *s_cnt = 0;
-    __syncthreads();
     int c = 0;
     for (int base = 0; base < len; base += B) {  // uniform
         const int off = base + tid;
