@@ -185,12 +185,16 @@ int g4s_server_create(const g4s_server_opts *opts, g4s_server **out)
     if (o.budget < 0 || o.chunk < 0) return fail(G4S_ERR_ARG, "bad server options");
     g4s_batch_opts bo{};
     bo.chunk = o.chunk > 0 ? o.chunk : 100;
-    // Serving defaults to 512-neuron slices, shorter than the batch default:
+    // Serving defaults to 384-neuron slices, shorter than the batch default:
     // at 50 requests per second (the batch a fraction of the device) a
     // request's steps are latency-bound, and more slices per instance run
-    // them faster -- p50 0.051 s and p99 0.131 s against 0.062 s and 0.156 s
-    // at 768, the same aggregate rate.
-    bo.slice_len = o.slice_len > 0 ? o.slice_len : 512;
+    // them faster.  Measured p50 / p99 at 50 requests per second: 0.043 /
+    // 0.122 s at 256, 0.048 / 0.120 s at 384, 0.051-0.053 / 0.131-0.139 s at
+    // 512, 0.062 / 0.156 s at 768, the same aggregate rate; at 400 requests
+    // per second (overload) 256 falls behind (1000 simulated seconds per
+    // second, p99 1.45 s) while 384 keeps up with 512 (1017 vs 1020, p99
+    // 0.92 vs 0.88 s).
+    bo.slice_len = o.slice_len > 0 ? o.slice_len : 384;
     int rc;
     g4s_batch *b = nullptr;
     if ((rc = g4s_batch_create(o.accum, &bo, &b))) return rc;
