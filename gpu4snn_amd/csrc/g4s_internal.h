@@ -25,9 +25,11 @@ constexpr uint32_t kSentinel = 0xFFFFFFFFu;   // empty spike-queue slot
 constexpr int kCtlStride = 4;                 // per-step control words
 enum { kCtlCount = 0 };  // spikes of the step; k_ap keeps finished workgroups in word 1
 constexpr uint32_t kSpinLimit = 1u << 24;     // bounded device waits (~10 s)
-// Grid-barrier arrival counters: one per group of workgroups (blockIdx % 8,
-// the XCD a workgroup is dealt to), each in its own 256-byte line.
-constexpr int kBarGroups = 8;
+// Grid-barrier arrival counters: one per group of workgroups (blockIdx % 16,
+// two groups per XCD), each in its own 256-byte line.  Measured at N=100k,
+// 241 workgroups: 16 counters 13.18-13.20 us per SKL step, 8 counters
+// 13.43-13.47, 32 counters 13.72 (det 17.7, 18.1, 17.7).
+constexpr int kBarGroups = 16;
 constexpr int kBarStride = 64;
 constexpr int kBarWords = kBarGroups * kBarStride;
 

@@ -232,11 +232,11 @@ __device__ __forceinline__ void push_flat(const DevNet &d, int t, int count, int
 }
 
 // Grid-wide barrier for cooperative launches.  Workgroup b arrives on
-// counter b % 8 (its XCD group): device-scope atomics on one address are
-// serialised, so eight counters in separate lines take an eighth of the
-// arrivals each.  Counters only increase; at epoch e group g is complete when
-// its counter reaches e * (number of workgroups in g).  Lanes 0..7 of the
-// first wave poll the eight counters together.  Bounded: returns false (and
+// counter b % kBarGroups (16 counters, two per XCD): device-scope atomics on
+// one address are serialised, so counters in separate lines each take a
+// share of the arrivals.  Counters only increase; at epoch e group g is
+// complete when its counter reaches e * (number of workgroups in g).  Lanes
+// 0..kBarGroups-1 of the first wave poll the counters together.  Bounded: returns false (and
 // raises the error flag) if the wait exceeds kSpinLimit or another
 // workgroup raised the flag.
 // It orders only data written and read with agent-scope atomics (the spike
