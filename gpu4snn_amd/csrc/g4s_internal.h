@@ -28,7 +28,8 @@ constexpr uint32_t kSpinLimit = 1u << 24;     // bounded device waits (~10 s)
 // Grid-barrier arrival counters: one per group of workgroups (blockIdx % 16,
 // two groups per XCD), each in its own 256-byte line.  Measured at N=100k,
 // 241 workgroups: 16 counters 13.18-13.20 us per SKL step, 8 counters
-// 13.43-13.47, 32 counters 13.72 (det 17.7, 18.1, 17.7).
+// 13.43-13.47, 12 counters 13.23-13.30, 24 counters 13.82, 32 counters
+// 13.72 (det 17.7, 18.1, 17.9, 17.85, 17.7).
 constexpr int kBarGroups = 16;
 constexpr int kBarStride = 64;
 constexpr int kBarWords = kBarGroups * kBarStride;
