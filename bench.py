@@ -88,7 +88,7 @@ def cpu_baseline(n, nsyn, seed, regime, steps):
 
 
 def batch_specs(count, seed):
-    """cfg3-style instances: N log-uniform on [1e3, 5e4], Nsyn from {100, 500,
+    """Instances of the batched workload: N log-uniform on [1e3, 5e4], Nsyn from {100, 500,
     1000}, independent seeds, thalamic scale U(0.8, 1.5)."""
     import numpy as np
     rng = np.random.default_rng(seed)

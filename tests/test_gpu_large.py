@@ -1,4 +1,4 @@
-"""GPU: configuration 2 -- one network of N=1,000,000 neurons, Nsyn=1000
+"""GPU: one network of N=1,000,000 neurons, Nsyn=1000
 (1e9 synapses, 8 GB), thalamic gains x1.5 (the irregular regime, 7.5/3),
 1000 steps with the persistent schemes AB and SKL in atomic mode.
 
@@ -37,7 +37,7 @@ def f64_oracle(oracle, big_net):
 
 
 @pytest.mark.parametrize("alg", ["AB", "SKL"])
-def test_cfg2_fp32_voltage_error(g4s, big_net, f64_oracle, alg):
+def test_n1e6_fp32_voltage_error(g4s, big_net, f64_oracle, alg):
     big_net.reset()
     # 256 slices requested (255 of 3936 neurons): SKL needs every slice co-resident
     words, counts = big_net.run(STEPS, alg=alg, accum="atomic", slices=256, block=1024)
@@ -49,7 +49,7 @@ def test_cfg2_fp32_voltage_error(g4s, big_net, f64_oracle, alg):
     same = np.all(own == gpu_sub, axis=0)
     err = np.abs(tr[-1] - v[LO:HI].astype(np.float64))
     p99 = float(np.percentile(err[same], 99))
-    print(f"\ncfg2 {alg}: rate {rate:.2f} Hz, {counts.mean():.0f} spikes/step; identical spike trains "
+    print(f"\nN=1e6 {alg}: rate {rate:.2f} Hz, {counts.mean():.0f} spikes/step; identical spike trains "
           f"{same.mean():.4f}; |dv| mean {err[same].mean():.2e} p99 {p99:.2e} max {err[same].max():.2e} mV")
     assert 5.0 < rate < 60.0
     assert same.mean() >= 0.80
@@ -57,7 +57,7 @@ def test_cfg2_fp32_voltage_error(g4s, big_net, f64_oracle, alg):
 
 
 @pytest.mark.parametrize("alg", ["AB", "SKL"])
-def test_cfg2_det_bitwise(g4s, oracle, big_net, alg):
+def test_n1e6_det_bitwise(g4s, oracle, big_net, alg):
     """Det mode at N=10^6 (255 slices of 3936 neurons, 1024 threads: the large
     plan, about 16k spikes per step in two passes) is bit-identical to the fp32
     oracle run on the same synapse table for the first 20 steps."""

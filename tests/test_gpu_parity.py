@@ -21,8 +21,8 @@ def _oracle_run(oracle, n, nsyn, steps, regime=1, seed=SEED, scale=1.0):
 
 
 @pytest.fixture(scope="module")
-def cfg0(oracle):
-    """Config 0: N=2500, Nsyn=1000, balanced regime, 1000 steps."""
+def net2500(oracle):
+    """N=2500, Nsyn=1000, balanced regime, 1000 steps."""
     return _oracle_run(oracle, 2500, 1000, 1000)
 
 
@@ -35,7 +35,7 @@ RATE_STEPS = 20_000
 
 
 @pytest.fixture(scope="module")
-def cfg0_rate(oracle):
+def net2500_rate(oracle):
     net = oracle.Network(2500, 1000, SEED)
     _, counts, _ = net.run(RATE_STEPS, raster=False)
     return int(counts.sum())
@@ -61,8 +61,8 @@ def test_device_thalamic_bitwise(g4s, oracle):
         assert np.array_equal(dev.view(np.uint32), ref.view(np.uint32)), t
 
 
-def test_params_and_table(g4s, cfg0):
-    onet = cfg0[0]
+def test_params_and_table(g4s, net2500):
+    onet = net2500[0]
     with g4s.Network(2500, 1000, SEED) as net:
         a, b, c, d = net.params()
         for x, y in ((a, onet.a), (b, onet.b), (c, onet.c), (d, onet.d)):
@@ -73,8 +73,8 @@ def test_params_and_table(g4s, cfg0):
 
 
 @pytest.mark.parametrize("alg", ALGS)
-def test_det_bitwise_cfg0(g4s, cfg0, alg):
-    onet, oraster, ocounts, olast = cfg0
+def test_det_bitwise_n2500(g4s, net2500, alg):
+    onet, oraster, ocounts, olast = net2500
     steps = oraster.shape[0]
     with g4s.Network(2500, 1000, SEED) as net:
         words, counts = net.run(steps, alg=alg, accum="det")
@@ -89,12 +89,12 @@ def test_det_bitwise_cfg0(g4s, cfg0, alg):
 
 
 @pytest.mark.parametrize("alg", ALGS)
-def test_atomic_cfg0(g4s, cfg0, cfg0_rate, alg):
-    onet, oraster, ocounts, _ = cfg0
+def test_atomic_n2500(g4s, net2500, net2500_rate, alg):
+    onet, oraster, ocounts, _ = net2500
     with g4s.Network(2500, 1000, SEED) as net:
         words, counts = net.run(100, alg=alg, accum="atomic")
         dense = g4s.raster_to_dense(words, 2500)
         assert np.array_equal(dense, oraster[:100]), "raster differs within the first 100 steps"
         _, more = net.run(RATE_STEPS - 100, alg=alg, accum="atomic", raster=False)
-        rate_gpu, rate_ref = int(counts.sum()) + int(more.sum()), cfg0_rate
+        rate_gpu, rate_ref = int(counts.sum()) + int(more.sum()), net2500_rate
         assert abs(rate_gpu - rate_ref) <= 0.01 * rate_ref, (rate_gpu, rate_ref)

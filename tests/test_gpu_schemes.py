@@ -116,17 +116,17 @@ def test_quiet_regime_and_zero_steps(g4s, oracle):
 
 @pytest.fixture(scope="module")
 def big():
-    """N=100,000, Nsyn=1000 (configuration 1), balanced regime."""
+    """N=100,000, Nsyn=1000, balanced regime."""
     return dict(n=100_000, nsyn=1000, seed=20221)
 
 
 @pytest.fixture(scope="module")
 def big_oracle(oracle, big):
-    """The oracle's first 60 steps of configuration 1."""
+    """The oracle's first 60 steps of the N=100,000 network."""
     return oracle_raster(oracle, big["n"], big["nsyn"], 60, big["seed"])
 
 
-def test_cfg1_det_all_schemes_identical_and_oracle(g4s, big, big_oracle):
+def test_n100k_det_all_schemes_identical_and_oracle(g4s, big, big_oracle):
     n, nsyn, seed = big["n"], big["nsyn"], big["seed"]
     steps = 60
     onet, oraster, _ = big_oracle
@@ -147,7 +147,7 @@ def test_cfg1_det_all_schemes_identical_and_oracle(g4s, big, big_oracle):
 
 
 @pytest.mark.parametrize("alg", ["AB", "SKL"])
-def test_cfg1_bench_plan_det_bitwise(g4s, big, big_oracle, alg):
+def test_n100k_bench_plan_det_bitwise(g4s, big, big_oracle, alg):
     """The tiling the bench times (256 slices requested, 1024 threads per
     workgroup) is bit-identical to the oracle in det mode."""
     n, nsyn, seed = big["n"], big["nsyn"], big["seed"]
@@ -161,7 +161,7 @@ def test_cfg1_bench_plan_det_bitwise(g4s, big, big_oracle, alg):
         assert np.array_equal(u.view(np.uint32), onet.u.view(np.uint32))
 
 
-def test_cfg1_atomic_rates(g4s, big):
+def test_n100k_atomic_rates(g4s, big):
     n, nsyn, seed = big["n"], big["nsyn"], big["seed"]
     with g4s.Network(n, nsyn, seed) as net:
         _, cdet = net.run(1000, alg="SKL", accum="det")
