@@ -24,8 +24,8 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # dominant kernel of each (scheme, mode): the one launch that carries a step
 KERNEL_OF = {("SKL", "atomic"): "k_skl_slice", ("AB", "atomic"): "k_slice_step", ("AP", "atomic"): "k_ap",
              ("N", "atomic"): "k_push_rows", ("S", "atomic"): "k_push_flat",
-             ("SKL", "det"): "k_skl_slice", ("AB", "det"): "k_slice_step", ("N", "det"): "k_pull_thread",
-             ("S", "det"): "k_pull_wave", ("AP", "det"): "k_ap_det"}
+             ("SKL", "det"): "k_skl_slice", ("AB", "det"): "k_slice_step", ("N", "det"): "k_slice_step",
+             ("S", "det"): "k_slice_step", ("AP", "det"): "k_slice_step"}
 
 
 def step_bytes(n, nsyn, spikes):

@@ -57,6 +57,12 @@ class Timing(C.Structure):
                 ("kernel_launches", C.c_int32), ("reserved", C.c_int32)]
 
 
+class LaunchInfo(C.Structure):
+    _fields_ = [("kernel", C.c_char * 64), ("grid", C.c_int32), ("block", C.c_int32), ("lds_bytes", C.c_int32),
+                ("vgprs", C.c_int32), ("sgprs", C.c_int32), ("waves_per_simd", C.c_int32),
+                ("resident_per_cu", C.c_int32), ("cus", C.c_int32), ("persistent", C.c_int32)]
+
+
 class BatchOpts(C.Structure):
     _fields_ = [("chunk", C.c_int32), ("slice_len", C.c_int32), ("graph", C.c_int32), ("schedule", C.c_int32),
                 ("budget", C.c_int64), ("block", C.c_int32), ("streams", C.c_int32)]
@@ -107,6 +113,9 @@ def lib():
             "g4s_set_regime": ([_vp, C.c_int, C.c_float], C.c_int),
             "g4s_run": ([_vp, C.c_int, C.c_int, C.c_int32, _vp, _vp, C.POINTER(RunOpts)], C.c_int),
             "g4s_get_timing": ([_vp, C.POINTER(Timing)], C.c_int),
+            "g4s_get_step_times": ([_vp, _vp, C.c_int32], C.c_int),
+            "g4s_get_launch_info": ([_vp, C.POINTER(LaunchInfo)], C.c_int),
+            "g4s_kernel_resources": ([C.c_char_p, C.c_char_p, _vp], C.c_int),
             "g4s_run_stream": ([_vp, C.c_int, C.c_int, C.c_int32, CHUNK_FN, _vp, C.POINTER(RunOpts)], C.c_int),
             "g4s_current_step": ([_vp], C.c_int64),
             "g4s_get_state": ([_vp, _vp, _vp, _vp, _vp], C.c_int),
@@ -248,6 +257,10 @@ class Network:
     def reset(self):
         _check(lib().g4s_reset(self._h))
 
+    def set_regime(self, regime, thal_scale=1.0):
+        """g4s_set_regime: new regime gains x thal_scale, connectivity kept."""
+        _check(lib().g4s_set_regime(self._h, int(regime), float(thal_scale)))
+
     def run(self, steps, alg="SKL", accum="atomic", raster=True, block=0, chunk=0, graph=0, slices=0,
             kernel_events=False):
         """Simulate `steps` steps; returns (raster words [steps, words] or None, counts)."""
@@ -291,6 +304,23 @@ class Network:
         _check(lib().g4s_get_timing(self._h, C.byref(t)))
         return {k: getattr(t, k) for k, _ in Timing._fields_}
 
+    def step_times(self):
+        """Device time of each step of the last run, ms (g4s_get_step_times)."""
+        t = self.timing()
+        out = np.zeros(max(0, int(t["steps"])), np.float32)
+        k = lib().g4s_get_step_times(self._h, _ptr(out), out.size)
+        if k < 0:
+            _check(k)
+        return out[:k]
+
+    def launch_info(self):
+        """Launch shape and occupancy of the last run's dominant kernel."""
+        li = LaunchInfo()
+        _check(lib().g4s_get_launch_info(self._h, C.byref(li)))
+        d = {k: getattr(li, k) for k, _ in LaunchInfo._fields_}
+        d["kernel"] = li.kernel.decode()
+        return d
+
     def state(self):
         v, u, I = (np.empty(self.n, np.float32) for _ in range(3))
         nos = np.empty(self.n, np.int32)
@@ -312,6 +342,14 @@ class Network:
         w = np.empty(self.n * self.nsyn, np.float32)
         _check(lib().g4s_get_synapses(self._h, _ptr(post), _ptr(w)))
         return post, w
+
+
+def kernel_resources(name, targs=""):
+    """The compiler's resource report of one kernel instantiation of this
+    build: waves per SIMD, VGPRs, AGPRs, SGPRs, static LDS, scratch per lane."""
+    out = np.zeros(6, np.int32)
+    _check(lib().g4s_kernel_resources(name.encode(), targs.encode(), out.ctypes.data))
+    return dict(zip(("waves_per_simd", "vgprs", "agprs", "sgprs", "lds", "scratch"), (int(x) for x in out)))
 
 
 def footprint(n, nsyn, chunk=0, slice_len=0, seed=0, regime=1, thal_scale=1.0):

@@ -42,6 +42,8 @@ constexpr int kMaxGroups = 4;         // schedule 0: instance groups on their ow
 // group.  3 is the default.
 constexpr int kDefaultGroups = 3;
 constexpr int kMaxSliceLen = 4096;
+// Queue schedule: a work unit packs its chunk step and slice in 16 bits each.
+constexpr int kMaxQueueField = 65535;
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
@@ -258,6 +260,9 @@ int g4s_batch_create(int accum, const g4s_batch_opts *opts, g4s_batch **out)
     if (o.block != 0 && o.block != 256 && o.block != 512 && o.block != 1024)
         return fail(G4S_ERR_ARG, "block must be 0, 256, 512 or 1024");
     if (o.schedule == 1 && o.block > 256) return fail(G4S_ERR_ARG, "the queue schedule runs 256-thread workgroups");
+    // the queue schedule packs a work unit as rel << 48 | p << 32 | slot
+    if (o.schedule == 1 && o.chunk > kMaxQueueField)
+        return fail(G4S_ERR_UNSUPPORTED, "the queue schedule needs chunk <= %d", kMaxQueueField);
     if (o.streams < 0 || o.streams > kMaxGroups) return fail(G4S_ERR_ARG, "streams must be 0..%d", kMaxGroups);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(G4S_ERR_HIP, "no HIP device");
@@ -294,6 +299,9 @@ int g4s_batch_add(g4s_batch *b, const g4s_spec *spec, int64_t steps, int keep_ra
     if (batch_lds(L, det, b->block) > 64 * 1024)
         return fail(G4S_ERR_UNSUPPORTED, "slice length %d needs %zu bytes of LDS", L, batch_lds(L, det, b->block));
     const int64_t P = (n + L - 1) / L;
+    if (b->opts.schedule == 1 && P > kMaxQueueField)
+        return fail(G4S_ERR_UNSUPPORTED, "the queue schedule needs at most %d slices per instance (N=%lld gives %lld)",
+                    kMaxQueueField, (long long)n, (long long)P);
     const int64_t fp = g4s_batch_footprint(spec, b->chunk, L);
     if (b->opts.budget > 0 && b->bytes_in_use + fp > b->opts.budget)
         return fail(G4S_ERR_BUDGET, "instance needs %lld bytes, %lld of %lld in use", (long long)fp,
@@ -481,6 +489,8 @@ int g4s_batch_step(g4s_batch *b, int32_t *active)
             const Inst &in = b->inst[(size_t)s];
             total += (size_t)in.d.P * (size_t)std::min<int64_t>(cs, in.steps - in.t);
         }
+        if (total > (size_t)0x7FFFFFFF)
+            return fail(G4S_ERR_UNSUPPORTED, "the queue schedule holds < 2^31 work units per chunk (%zu)", total);
         if ((rc = grow_dev(b, &b->d_units, &b->h_units, &b->units_cap, total))) return rc;
         size_t dn = b->done_cap;
         if ((rc = grow_dev<int32_t>(b, &b->d_done, nullptr, &dn, b->inst.size()))) return rc;

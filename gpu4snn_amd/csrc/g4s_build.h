@@ -9,8 +9,4 @@ int host_threads(int requested);
 // Outgoing table, row-major [n][nsyn] of {target, weight bits}, rows sorted.
 void build_table(uint64_t seed, int64_t n, int64_t n_exc, int32_t nsyn, uint2 *tab, int threads);
 
-// Incoming table: in_ptr[n+1], in_tab[n*nsyn] of {source, weight bits},
-// ascending source within each target.
-void build_incoming(int64_t n, int32_t nsyn, const uint2 *tab, int64_t *in_ptr, uint2 *in_tab, int threads);
-
 }  // namespace g4s

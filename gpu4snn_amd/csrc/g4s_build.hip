@@ -74,54 +74,4 @@ void build_table(uint64_t seed, int64_t n, int64_t n_exc, int32_t nsyn, uint2 *t
     });
 }
 
-// Incoming table: for every target j the pairs {source, weight} in ascending
-// source order.  Thread k owns the source range R_k; counts per (k, j) give
-// each thread its write cursor, so the fill is parallel and the order within
-// a target is ascending because ranges are ascending in k.
-void build_incoming(int64_t n, int32_t nsyn, const uint2 *tab, int64_t *in_ptr, uint2 *in_tab, int threads)
-{
-    const int T = std::min<int64_t>(host_threads(threads), std::max<int64_t>(1, n / 1024));
-    const int64_t per = (n + T - 1) / T;
-    std::vector<std::vector<int64_t>> cur((size_t)T, std::vector<int64_t>((size_t)n, 0));
-    {
-        std::vector<std::thread> pool;
-        for (int k = 0; k < T; ++k)
-            pool.emplace_back([&, k] {
-                const int64_t lo = k * per, hi = std::min<int64_t>(n, lo + per);
-                auto &c = cur[(size_t)k];
-                for (int64_t i = lo; i < hi; ++i) {
-                    const uint2 *row = tab + (size_t)i * (size_t)nsyn;
-                    for (int32_t e = 0; e < nsyn; ++e) c[row[e].x] += 1;
-                }
-            });
-        for (auto &th : pool) th.join();
-    }
-    int64_t acc = 0;
-    for (int64_t j = 0; j < n; ++j) {
-        in_ptr[j] = acc;
-        for (int k = 0; k < T; ++k) {
-            const int64_t c = cur[(size_t)k][(size_t)j];
-            cur[(size_t)k][(size_t)j] = acc;
-            acc += c;
-        }
-    }
-    in_ptr[n] = acc;
-    {
-        std::vector<std::thread> pool;
-        for (int k = 0; k < T; ++k)
-            pool.emplace_back([&, k] {
-                const int64_t lo = k * per, hi = std::min<int64_t>(n, lo + per);
-                auto &c = cur[(size_t)k];
-                for (int64_t i = lo; i < hi; ++i) {
-                    const uint2 *row = tab + (size_t)i * (size_t)nsyn;
-                    for (int32_t e = 0; e < nsyn; ++e) {
-                        const uint32_t j = row[e].x;
-                        in_tab[(size_t)c[j]++] = make_uint2((uint32_t)i, row[e].y);
-                    }
-                }
-            });
-        for (auto &th : pool) th.join();
-    }
-}
-
 }  // namespace g4s

@@ -14,12 +14,14 @@
 // Optional flags select a subset or change the accumulation mode/seed.
 // Spike rows are streamed to spiketimes.csv chunk by chunk while the device
 // runs the next chunk (g4s_run_stream).  Per-step times in computetime.csv
-// are the run's device time divided by its steps (steps run in
-// device-resident chunks).
+// are device times of each step (g4s_get_step_times), and after each run
+// the dominant kernel's launch shape and occupancy is printed, as the
+// reference prints its kernels' (cu:800-838).
 #include "gpu4snn.h"
 
 #include <sys/stat.h>
 
+#include <algorithm>
 #include <cerrno>
 #include <cstdio>
 #include <cstdlib>
@@ -212,9 +214,24 @@ int main(int argc, char **argv)
             g4s_get_timing(net, &tm);
             const double dev_ms = iters > 0 ? tm.device_ms : 0.0, wall_ms = iters > 0 ? tm.wall_ms : 0.0;
             const long long total = sink.total;
-            const double per = iters > 0 ? dev_ms / (double)iters : 0.0;
+            // TimeKernelUpdate: the step's device time (start to start of
+            // consecutive steps on the device clock); TimeSpent adds the
+            // step's share of what the run spent outside the device (chunk
+            // copies, spiketimes.csv writes).
+            std::vector<float> step_ms((size_t)std::max<long>(iters, 0L), 0.0f);
+            if (iters > 0 && g4s_get_step_times(net, step_ms.data(), (int32_t)iters) < 0)
+                check(G4S_ERR_STATE, "g4s_get_step_times");
+            const double host_per = iters > 0 ? std::max(0.0, wall_ms - dev_ms) / (double)iters : 0.0;
             for (size_t r = 0; r < sink.counts.size(); ++r)
-                ctime << r << " " << sink.counts[r] << " " << per << " " << per << "\n";
+                ctime << r << " " << sink.counts[r] << " " << step_ms[r] << " " << step_ms[r] + host_per << "\n";
+            if (iters > 0) {
+                g4s_launch_info li{};
+                g4s_get_launch_info(net, &li);
+                printf("Launched %s: grid %d, block %d, LDS %d bytes per workgroup, %d VGPRs, %d SGPRs, "
+                       "%d waves per SIMD, %d workgroups per CU of %d CUs%s\n",
+                       li.kernel, li.grid, li.block, li.lds_bytes, li.vgprs, li.sgprs, li.waves_per_simd,
+                       li.resident_per_cu, li.cus, li.persistent ? " (persistent: grid within residency)" : "");
+            }
             std::cout << "\nEnd of simulation...\n";
             std::cout << "Simulation took: " << wall_ms << " ms.\n";
             std::cout << "Without data transfers: " << dev_ms << " ms.\n";

@@ -23,7 +23,9 @@ int build_synapses(const g4s_spec *spec, uint2 *syn, hipStream_t stream);
 
 constexpr uint32_t kSentinel = 0xFFFFFFFFu;   // empty spike-queue slot
 constexpr int kCtlStride = 4;                 // per-step control words
-enum { kCtlCount = 0 };  // spikes of the step; k_ap keeps finished workgroups in word 1
+// Control row of a step: [0] spikes of the step (k_ap keeps finished
+// workgroups in word 1), [2..3] the step's start on the device wall clock.
+enum { kCtlCount = 0, kCtlStamp = 2 };
 constexpr uint32_t kSpinLimit = 1u << 24;     // bounded device waits (~10 s)
 // Grid-barrier arrival counters: one per group of workgroups (blockIdx % 16,
 // two groups per XCD), each in its own 256-byte line.  Measured at N=100k,
@@ -40,12 +42,11 @@ constexpr int kBarWords = kBarGroups * kBarStride;
 //    stream);
 //  - syn is the outgoing table: row i = Nsyn {target, weight-bits} pairs,
 //    ascending target, row offset i*Nsyn (64-bit);
-//  - in_ptr/in_syn is the incoming table (det mode): entries {source,
-//    weight-bits} of target j in ascending source order;
 //  - isyn0/isyn1, bm0/bm1, q0/q1 are double-buffered by step parity: step t
 //    reads input buffer t&1 and scatters into (t+1)&1, so propagation of
 //    step t can never reach a neuron's step-t update;
-//  - ctl holds [count, head, done, -] per step of the current chunk.
+//  - ctl holds a control row per step of the current chunk (kCtl*), plus
+//    one more row for the chunk's end stamp.
 struct DevNet {
     int32_t n, n_exc, nsyn, words;
     uint64_t seed;
@@ -54,8 +55,6 @@ struct DevNet {
     float *isyn0, *isyn1;
     int32_t *nospks;
     const uint2 *syn;
-    const int64_t *in_ptr;
-    const uint2 *in_syn;
     uint32_t *bm0, *bm1;
     uint32_t *q0, *q1;
     int32_t *ctl;
@@ -88,13 +87,14 @@ struct Launch {
     hipStream_t stream;
     // Optional: events stamped by the dispatch itself at the kernel's start
     // and end (hipExtLaunchKernelGGL) -- the dominant kernel's timing.
-    // Cooperative launches ignore them.
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
 };
 
 // Kernel launchers (g4s_kernels.hip).  Each returns the HIP status.
 hipError_t launch_init(const DevNet &d, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, size_t n, uint32_t val, hipStream_t s);
+// Stamp the end of a chunk of `rel` steps into control row `rel`.
+hipError_t launch_stamp(int32_t *ctl, int rel, hipStream_t s);
 hipError_t launch_update(const DevNet &d, int t, int rel, int last, const Launch &L);
 hipError_t launch_push_rows(const DevNet &d, int t, int rel, const Launch &L);
 // Scatter every row set in step t's bitmap into isyn[(t+1)&1] (rebuild of
@@ -110,8 +110,13 @@ hipError_t launch_slice_offsets(const DevNet &d, uint16_t *soff, hipStream_t s);
 hipError_t launch_slice_step(const DevNet &d, int rel, bool det, const Launch &L);
 hipError_t launch_skl_slice(const DevNet &d, int t0, int steps, int last_t, bool det, const Launch &L);
 // Co-resident workgroups per CU of the SKL slice kernel used for slice
-// length L at `block` threads.
+// length L at `block` threads (resident_per_cu of that instantiation).
 int skl_slice_blocks_per_cu(int L, int block, bool det);
+// Template arguments ("R,det,large") of the slice kernels (k_slice_step,
+// k_skl_slice) launched for slice length L at `block` threads, and the
+// workgroups of k_slice_step one CU holds at once.
+void slice_kernel_targs(int L, int block, bool det, char *out, size_t cap);
+int slice_step_blocks_per_cu(int L, int block, bool det);
 // Dynamic LDS bytes of the slice kernels (single network at `block`
 // threads; batched at 256).
 // LDS of one workgroup on MI355X (gfx950): single-network slice kernels run
@@ -119,9 +124,6 @@ int skl_slice_blocks_per_cu(int L, int block, bool det);
 constexpr size_t kSliceLdsMax = 160 * 1024;
 size_t slice_lds(int L, int block, bool det);
 size_t batch_lds(int L, bool det, int block = 256);
-hipError_t launch_pull_thread(const DevNet &d, int t, int rel, int last, const Launch &L);
-hipError_t launch_pull_wave(const DevNet &d, int t, int rel, int last, const Launch &L);
-hipError_t launch_ap_det(const DevNet &d, int t, int rel, int last, const Launch &L);
 // Batched step `rel` of a chunk: one workgroup per work unit, grid = units
 // (padded entries exit).  L.block = 256, 512 or 1024.
 hipError_t launch_batch_step(const DevNet *descs, const uint2 *work, int grid, int rel, bool det, size_t lds,
@@ -144,9 +146,28 @@ hipError_t launch_thalamic(uint64_t seed, uint32_t t, int64_t n, float *out, hip
 hipError_t launch_normal_words(const uint32_t *x0, const uint32_t *x1, int64_t n, float *out,
                                hipStream_t s);
 
-// Co-resident workgroups of each cooperative kernel at `block` threads.
-enum CoopKernel { kCoopAP, kCoopAPDet };
-int coop_blocks_per_cu(CoopKernel k, int block, size_t shmem);
+// Resource usage of one kernel instantiation as the compiler reported it
+// for gfx950 (generated at build time into g4s_resources.cpp): name, template
+// arguments ("1,false,false"), occupancy in waves per SIMD (limited by the
+// kernel's VGPRs, AGPRs and SGPRs), registers, static LDS bytes per
+// workgroup and scratch bytes per lane.
+struct KernelRes {
+    const char *name, *targs;
+    int occupancy, vgprs, agprs, sgprs, lds, scratch;
+};
+extern const KernelRes kKernelRes[];
+extern const int kKernelResCount;
+const KernelRes *kernel_res(const char *name, const char *targs);
+// Workgroups of `block` threads and `dyn_lds` bytes of dynamic LDS that one
+// CU holds at once: the compiler's occupancy over the workgroup's waves per
+// SIMD, the CU's 160 KB of LDS, and the occupancy API, whichever is least
+// (the API alone overstates kernels whose SGPRs limit them, e.g. 106 SGPRs
+// hold 7 waves per SIMD, not 8).  0 if the kernel is not in the table.
+int resident_per_cu(const char *name, const char *targs, const void *fn, int block, size_t dyn_lds);
+
+// Co-resident workgroups of each persistent kernel at `block` threads.
+enum PersistentKernel { kPersistAP };
+int persistent_blocks_per_cu(PersistentKernel k, int block);
 int device_cus();
 
 }  // namespace g4s

@@ -10,15 +10,19 @@
 // Two accumulation modes:
 //   atomic: (c) scatters fp32 atomics into isyn[(t+1)&1] (reference order:
 //           none);
-//   det:    (c) is fused into the next step's (a): each neuron gathers its
-//           incoming synapses in ascending source order from the previous
-//           step's bitmap, summing from 0.0f.  The result equals a sequential
-//           push in ascending spike order, bit for bit.
+//   det:    (c) is fused into the next step's (a): each slice of neurons
+//           gathers the segments of the previous step's spiking rows that
+//           target it and sums every neuron's input in ascending source
+//           order from 0.0f (every scheme; see slice_accumulate).  The result
+//           equals a sequential push in ascending spike order, bit for bit.
 // Compiled with -ffp-contract=off: the neuron arithmetic matches the host.
 #include "g4s_internal.h"
 
-#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_ext.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
 
 namespace g4s {
 
@@ -41,6 +45,17 @@ __device__ __forceinline__ int ld_agent(const int32_t *p)
 
 __device__ __forceinline__ bool error_set(const DevNet &d) { return ld_agent(d.err) != 0; }
 
+// Start of step `rel` of a chunk on the device's constant-rate wall clock,
+// in words 2-3 of the step's control row; row `cs` (one past the chunk's
+// last step) receives the chunk's end.  Workgroup 0 of the step's first
+// kernel stamps it, so consecutive stamps bracket one step.
+__device__ __forceinline__ void stamp_step(int32_t *ctl, int rel)
+{
+    const uint64_t c = (uint64_t)wall_clock64();
+    ctl[rel * kCtlStride + kCtlStamp] = (int32_t)(uint32_t)c;
+    ctl[rel * kCtlStride + kCtlStamp + 1] = (int32_t)(uint32_t)(c >> 32);
+}
+
 __device__ __forceinline__ void raise_error(const DevNet &d)
 {
     __hip_atomic_store(d.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -62,35 +77,6 @@ __device__ __forceinline__ bool neuron_step(const Net &d, int t, int j, float is
     if (spk) d.nospks[j] += 1;
     if (last) d.lastI[j] = I;
     return spk;
-}
-
-// Spike output of one wave covering neurons [wave_base, wave_base+64):
-// two bitmap words (always written, so stale bits are cleared), the raster
-// row of the chunk, and the queue entries appended at one atomic per wave.
-// Waves starting at or beyond `limit` (the end of the caller's neuron range)
-// write nothing: their words belong to another workgroup.
-__device__ __forceinline__ void emit(const DevNet &d, int t, int rel, int wave_base, bool spk, int j,
-                                     int limit)
-{
-    if (wave_base >= limit) return;  // uniform over the wave
-    const uint64_t mask = __ballot(spk);
-    const int lane = lane_id();
-    uint32_t *bm = (t & 1) ? d.bm1 : d.bm0;
-    const int w = (wave_base >> 5) + lane;
-    if (lane < 2 && w < d.words && (w << 5) < limit) {
-        const uint32_t bits = lane ? (uint32_t)(mask >> 32) : (uint32_t)mask;
-        bm[w] = bits;
-        if (d.raster) d.raster[(size_t)rel * (size_t)d.words + (size_t)w] = bits;
-    }
-    if (mask) {
-        int base = 0;
-        if (lane == 0) base = atomicAdd(&d.ctl[rel * kCtlStride + kCtlCount], __popcll(mask));
-        base = __shfl(base, 0);
-        if (spk) {
-            uint32_t *q = (t & 1) ? d.q1 : d.q0;
-            q[base + bits_below(mask)] = (uint32_t)j;
-        }
-    }
 }
 
 // Spike output of one workgroup tile [base, base + blockDim.x): each wave's
@@ -155,39 +141,6 @@ __device__ __forceinline__ float take_input(const DevNet &d, int t, int j)
     const float x = in[j];
     in[j] = 0.0f;
     return x;
-}
-
-__device__ __forceinline__ bool spiked(const uint32_t *bm, uint32_t i)
-{
-    return (bm[i >> 5] >> (i & 31u)) & 1u;
-}
-
-// Ordered gather of neuron j's input from the previous step's spikes:
-// incoming synapses in ascending source id, summed from 0.0f.  Loads are
-// issued eight at a time; the additions stay in order.
-__device__ __forceinline__ float gather_input(const DevNet &d, int t, int j)
-{
-    if (t == 0) return 0.0f;
-    const uint32_t *bm = ((t - 1) & 1) ? d.bm1 : d.bm0;
-    const int64_t e0 = d.in_ptr[j], e1 = d.in_ptr[j + 1];
-    float acc = 0.0f;
-    int64_t e = e0;
-    for (; e + 8 <= e1; e += 8) {
-        uint2 s[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) s[k] = d.in_syn[e + k];
-        bool h[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) h[k] = spiked(bm, s[k].x);
-#pragma unroll
-        for (int k = 0; k < 8; ++k)
-            if (h[k]) acc = acc + u2f(s[k].y);
-    }
-    for (; e < e1; ++e) {
-        const uint2 s = d.in_syn[e];
-        if (spiked(bm, s.x)) acc = acc + u2f(s.y);
-    }
-    return acc;
 }
 
 // One wave scatters row s of the outgoing table into `out`.  Queue entries
@@ -323,9 +276,16 @@ __global__ void __launch_bounds__(256) k_fill_u32(uint32_t *p, size_t n, uint32_
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = val;
 }
 
+// The chunk's end stamp (control row `rel` = the chunk's length).
+__global__ void __launch_bounds__(64) k_stamp(int32_t *ctl, int rel)
+{
+    if (threadIdx.x == 0) stamp_step(ctl, rel);
+}
+
 // (a)+(b) for the atomic-mode per-step schemes N, S, AB.
 __global__ void __launch_bounds__(256) k_update(DevNet d, int t, int rel, int last)
 {
+    if (blockIdx.x == 0 && threadIdx.x == 0) stamp_step(d.ctl, rel);
     const int stride = gridDim.x * blockDim.x;
     for (int base = blockIdx.x * blockDim.x; base < d.n; base += stride) {
         const int j = base + (int)threadIdx.x;
@@ -398,6 +358,7 @@ __global__ void __launch_bounds__(256) k_push_flat(DevNet d, int t, int rel)
 // valid for whichever scheme runs next.  Workgroups of at most 256 threads.
 __global__ void __launch_bounds__(256) k_ap(DevNet d, int t, int rel, int last, int prev_count)
 {
+    if (blockIdx.x == 0 && threadIdx.x == 0) stamp_step(d.ctl, rel);
     const int stride = gridDim.x * blockDim.x;
     {
         uint32_t *qn = ((t + 1) & 1) ? d.q1 : d.q0;
@@ -1074,6 +1035,7 @@ __global__ void __launch_bounds__(1024) k_slice_step(DevNet d, int rel)
     const int t = d.tinfo[0] + rel;
     const bool last = t == d.tinfo[1];
     if (threadIdx.x == 0) s_cnt = 0;  // slice_accumulate's barriers precede slice_count
+    if (blockIdx.x == 0 && threadIdx.x == 0) stamp_step(d.ctl, rel);
     const int p = slice_of_block((int)blockIdx.x, (int)gridDim.x);
     const int lo = p * d.L;
     const int len = min(d.L, d.n - lo);
@@ -1321,6 +1283,7 @@ __global__ void __launch_bounds__(1024) k_skl_slice(DevNet d, int t0, int steps,
     uint32_t epoch = 0;
     for (int rel = 0; rel < steps; ++rel) {
         const int t = t0 + rel;
+        if (blockIdx.x == 0 && tid == 0) stamp_step(d.ctl, rel);
         // det mode computes the step's thalamic draws while the accumulation's
         // first bitmap loads are in flight (18.25 -> 18.0 us per step at
         // N=100k); in atomic mode that ran slower (13.35 -> 13.8), and the
@@ -1374,77 +1337,6 @@ __global__ void __launch_bounds__(1024) k_skl_slice(DevNet d, int t0, int steps,
             d.u[j] = u[k];
             d.nospks[j] += nos[k];
         }
-    }
-}
-
-// det mode, N scheme: thread per neuron gathers its input and updates.
-__global__ void __launch_bounds__(256) k_pull_thread(DevNet d, int t, int rel, int last)
-{
-    const int stride = gridDim.x * blockDim.x;
-    for (int base = blockIdx.x * blockDim.x; base < d.n; base += stride) {
-        const int j = base + (int)threadIdx.x;
-        bool spk = false;
-        if (j < d.n) spk = neuron_step(d, t, j, gather_input(d, t, j), last != 0);
-        emit_tile<false>(d, t, rel, base, spk, j, false);
-    }
-}
-
-// det mode, S scheme: synapse-parallel gather.  A wave owns 64 neurons; for
-// each of them the 64 lanes load 64 incoming synapses at a time, test the
-// source bits and the hits are added in lane (= ascending source) order.
-__global__ void __launch_bounds__(256) k_pull_wave(DevNet d, int t, int rel, int last)
-{
-    const int lane = lane_id();
-    const uint32_t *bm = ((t - 1) & 1) ? d.bm1 : d.bm0;
-    const int waves = gridDim.x * (blockDim.x / kWave);
-    for (int wb = (blockIdx.x * (blockDim.x / kWave) + (int)(threadIdx.x / kWave)) * kWave; wb < d.n;
-         wb += waves * kWave) {
-        float mine = 0.0f;
-        if (t > 0) {
-            const int nj = min(kWave, d.n - wb);
-            for (int p = 0; p < nj; ++p) {
-                const int j = wb + p;
-                const int64_t e0 = d.in_ptr[j], e1 = d.in_ptr[j + 1];
-                float acc = 0.0f;
-                for (int64_t e = e0; e < e1; e += kWave) {
-                    const int64_t my = e + lane;
-                    uint2 s = make_uint2(0u, 0u);
-                    bool hit = false;
-                    if (my < e1) {
-                        s = d.in_syn[my];
-                        hit = spiked(bm, s.x);
-                    }
-                    uint64_t m = __ballot(hit);
-                    while (m) {
-                        const int k = __builtin_ctzll(m);
-                        acc = acc + __builtin_bit_cast(float, __builtin_amdgcn_readlane((int)s.y, k));
-                        m &= m - 1;
-                    }
-                }
-                if (lane == p) mine = acc;
-            }
-        }
-        const int j = wb + lane;
-        bool spk = false;
-        if (j < d.n) spk = neuron_step(d, t, j, mine, last != 0);
-        emit(d, t, rel, wb, spk, j, d.n);
-    }
-}
-
-// det mode, AP scheme: one co-resident launch per step; wave w of W takes
-// the 64-neuron tiles w, w + W, w + 2W, ...  (A shared atomic tile counter
-// measured slower: its device-scope atomics on one address serialise.)
-__global__ void __launch_bounds__(256) k_ap_det(DevNet d, int t, int rel, int last)
-{
-    const int lane = lane_id();
-    const int tiles = (d.n + kWave - 1) / kWave;
-    const int nwave = (int)blockDim.x / kWave;
-    const int waves = (int)gridDim.x * nwave;
-    for (int tile = (int)blockIdx.x * nwave + (int)threadIdx.x / kWave; tile < tiles; tile += waves) {
-        const int j = tile * kWave + lane;
-        bool spk = false;
-        if (j < d.n) spk = neuron_step(d, t, j, gather_input(d, t, j), last != 0);
-        emit(d, t, rel, tile * kWave, spk, j, d.n);
     }
 }
 
@@ -1639,11 +1531,6 @@ hipError_t plain_launch(F kernel, const Launch &L, uint32_t shmem, Args... args)
     return hipGetLastError();
 }
 
-hipError_t coop(const void *fn, const Launch &L, void **args, size_t shmem = 0)
-{
-    return hipLaunchCooperativeKernel(fn, dim3(L.grid), dim3(L.block), args, shmem, L.stream);
-}
-
 // The large plan exists for R >= 2 (slices of kLargeSlice neurons or more
 // at up to 1024 threads).
 template <int R, bool kDet>
@@ -1654,13 +1541,18 @@ const void *skl_slice_fn_r(bool large)
     return (const void *)k_skl_slice<R, kDet, false>;
 }
 
+// The SKL grid waits on itself (one grid barrier per step), so every
+// workgroup must be resident at once: the runtime sizes it from
+// skl_slice_blocks_per_cu.  It is an ordinary launch: the barrier is the
+// library's own, and its waits are bounded.
 template <int R, bool kDet>
 hipError_t launch_skl_slice_r(const DevNet &d, int t0, int steps, int last_t, const Launch &L)
 {
-    DevNet dd = d;
-    void *args[] = {&dd, &t0, &steps, &last_t};
-    return coop(skl_slice_fn_r<R, kDet>(large_slice_plan(d.L, L.block, kDet)), L, args,
-                single_slice_lds(d.L, L.block, kDet));
+    const uint32_t lds = (uint32_t)single_slice_lds(d.L, L.block, kDet);
+    if constexpr (R >= 2)
+        if (large_slice_plan(d.L, L.block, kDet))
+            return plain_launch(k_skl_slice<R, kDet, true>, L, lds, d, t0, steps, last_t);
+    return plain_launch(k_skl_slice<R, kDet, false>, L, lds, d, t0, steps, last_t);
 }
 
 template <int R, bool kDet>
@@ -1688,6 +1580,12 @@ hipError_t launch_init(const DevNet &d, hipStream_t s)
 {
     const int64_t m = d.n > d.words ? d.n : d.words;
     hipLaunchKernelGGL(k_init, dim3(grid_for(m, 256)), dim3(256), 0, s, d);
+    return hipGetLastError();
+}
+
+hipError_t launch_stamp(int32_t *ctl, int rel, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, s, ctl, rel);
     return hipGetLastError();
 }
 
@@ -1729,9 +1627,9 @@ hipError_t launch_push_flat(const DevNet &d, int t, int rel, const Launch &L)
 
 hipError_t launch_ap(const DevNet &d, int t, int rel, int last, int prev_count, const Launch &L)
 {
-    DevNet dd = d;
-    void *args[] = {&dd, &t, &rel, &last, &prev_count};
-    return coop((const void *)k_ap, L, args);
+    // consumers wait on producers of the same grid: the runtime sizes it
+    // from persistent_blocks_per_cu, so every workgroup is resident at once
+    return plain_launch(k_ap, L, 0, d, t, rel, last, prev_count);
 }
 
 hipError_t launch_slice_offsets(const DevNet &d, uint16_t *soff, hipStream_t s)
@@ -1775,15 +1673,53 @@ hipError_t launch_skl_slice(const DevNet &d, int t0, int steps, int last_t, bool
     return det ? launch_skl_slice_t<true>(d, t0, steps, last_t, L) : launch_skl_slice_t<false>(d, t0, steps, last_t, L);
 }
 
+void slice_kernel_targs(int L, int block, bool det, char *out, size_t cap)
+{
+    const int r = (L + block - 1) / block;
+    const int R = r <= 1 ? 1 : r <= 2 ? 2 : r <= 4 ? 4 : r <= 8 ? 8 : 16;
+    const bool large = R >= 2 && large_slice_plan(L, block, det);
+    snprintf(out, cap, "%d,%s,%s", R, det ? "true" : "false", large ? "true" : "false");
+}
+
 int skl_slice_blocks_per_cu(int L, int block, bool det)
 {
-    int per_cu = 0;
     const int r = (L + block - 1) / block;
-    const bool large = large_slice_plan(L, block, det);
+    const bool large = r >= 2 && large_slice_plan(L, block, det);
     const void *fn = det ? skl_slice_fn<true>(r, large) : skl_slice_fn<false>(r, large);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, single_slice_lds(L, block, det)) != hipSuccess)
-        return 0;
-    return per_cu;
+    char targs[32];
+    slice_kernel_targs(L, block, det, targs, sizeof(targs));
+    return resident_per_cu("k_skl_slice", targs, fn, block, single_slice_lds(L, block, det));
+}
+
+template <int R, bool kDet>
+const void *slice_step_fn_r(bool large)
+{
+    if constexpr (R >= 2)
+        if (large) return (const void *)k_slice_step<R, kDet, true>;
+    return (const void *)k_slice_step<R, kDet, false>;
+}
+
+int slice_step_blocks_per_cu(int L, int block, bool det)
+{
+    const int r = (L + block - 1) / block;
+    const bool large = r >= 2 && large_slice_plan(L, block, det);
+    const int R = r <= 1 ? 1 : r <= 2 ? 2 : r <= 4 ? 4 : r <= 8 ? 8 : 16;
+    const void *fn = nullptr;
+    switch (R * 2 + (det ? 1 : 0)) {
+    case 2: fn = slice_step_fn_r<1, false>(large); break;
+    case 3: fn = slice_step_fn_r<1, true>(large); break;
+    case 4: fn = slice_step_fn_r<2, false>(large); break;
+    case 5: fn = slice_step_fn_r<2, true>(large); break;
+    case 8: fn = slice_step_fn_r<4, false>(large); break;
+    case 9: fn = slice_step_fn_r<4, true>(large); break;
+    case 16: fn = slice_step_fn_r<8, false>(large); break;
+    case 17: fn = slice_step_fn_r<8, true>(large); break;
+    case 32: fn = slice_step_fn_r<16, false>(large); break;
+    default: fn = slice_step_fn_r<16, true>(large); break;
+    }
+    char targs[32];
+    slice_kernel_targs(L, block, det, targs, sizeof(targs));
+    return resident_per_cu("k_slice_step", targs, fn, block, single_slice_lds(L, block, det));
 }
 
 size_t slice_lds(int L, int block, bool det) { return single_slice_lds(L, block, det); }
@@ -1792,24 +1728,6 @@ size_t batch_lds(int L, bool det, int block)
     return slice_lds_bytes(L, block, det, det ? entries_batch<true>() : entries_batch<false>(),
                            det ? spikes_batch<true>() : spikes_batch<false>(), kOwnerBatch);
 }
-
-hipError_t launch_pull_thread(const DevNet &d, int t, int rel, int last, const Launch &L)
-{
-    return plain_launch(k_pull_thread, L, 0, d, t, rel, last);
-}
-
-hipError_t launch_pull_wave(const DevNet &d, int t, int rel, int last, const Launch &L)
-{
-    return plain_launch(k_pull_wave, L, 0, d, t, rel, last);
-}
-
-hipError_t launch_ap_det(const DevNet &d, int t, int rel, int last, const Launch &L)
-{
-    DevNet dd = d;
-    void *args[] = {&dd, &t, &rel, &last};
-    return coop((const void *)k_ap_det, L, args);
-}
-
 
 hipError_t launch_batch_chunk(const DevNet *descs, unsigned long long *units, int32_t *ctr, int32_t *done, int total,
                               bool det, size_t lds, const Launch &L)
@@ -1871,16 +1789,34 @@ hipError_t launch_normal_words(const uint32_t *x0, const uint32_t *x1, int64_t n
     return hipGetLastError();
 }
 
-int coop_blocks_per_cu(CoopKernel k, int block, size_t shmem)
+const KernelRes *kernel_res(const char *name, const char *targs)
 {
-    const void *fn = nullptr;
+    for (int k = 0; k < kKernelResCount; ++k)
+        if (strcmp(kKernelRes[k].name, name) == 0 && strcmp(kKernelRes[k].targs, targs) == 0) return &kKernelRes[k];
+    return nullptr;
+}
+
+int resident_per_cu(const char *name, const char *targs, const void *fn, int block, size_t dyn_lds)
+{
+    constexpr int64_t kLdsPerCu = 160 * 1024, kLdsGranule = 1024, kSimdsPerCu = 4;
+    const KernelRes *r = kernel_res(name, targs);
+    if (!r || r->occupancy <= 0 || block <= 0) return 0;
+    const int waves = (block + kWave - 1) / kWave;
+    const int per_simd = (int)((waves + kSimdsPerCu - 1) / kSimdsPerCu);  // a workgroup's waves on each SIMD
+    int64_t n = r->occupancy / per_simd;
+    const int64_t lds = (int64_t)dyn_lds + r->lds;
+    if (lds > 0) n = std::min<int64_t>(n, kLdsPerCu / ((lds + kLdsGranule - 1) / kLdsGranule * kLdsGranule));
+    int api = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&api, fn, block, dyn_lds) != hipSuccess) return 0;
+    return (int)std::max<int64_t>(0, std::min<int64_t>(n, api));
+}
+
+int persistent_blocks_per_cu(PersistentKernel k, int block)
+{
     switch (k) {
-    case kCoopAP: fn = (const void *)k_ap; break;
-    case kCoopAPDet: fn = (const void *)k_ap_det; break;
+    case kPersistAP: return resident_per_cu("k_ap", "", (const void *)k_ap, block, 0);
     }
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, shmem) != hipSuccess) return 0;
-    return per_cu;
+    return 0;
 }
 
 int device_cus()

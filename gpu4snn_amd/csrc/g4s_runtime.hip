@@ -22,6 +22,7 @@ namespace {
 thread_local std::string g_err;
 }  // namespace
 
+
 namespace g4s {
 
 int fail(int code, const char *fmt, ...)
@@ -97,7 +98,6 @@ struct g4s_net {
     uint32_t *h_raster[2] = {nullptr, nullptr};
     int32_t *h_ctl[2] = {nullptr, nullptr};
     int32_t *h_err = nullptr;      // [2]
-    bool det_ready = false;
     bool slices_ready = false;
     // Per-step AB launches captured once per chunk length and device layout.
     hipGraph_t graph = nullptr;
@@ -111,9 +111,14 @@ struct g4s_net {
     bool isyn_valid = true;
     int32_t last_count = 0;     // spikes of step t-1
     int64_t est_count = 0;      // recent max spikes per step (grid sizing)
-    int coop_cache[8][5] = {};  // co-resident blocks per CU per (kernel, block/64)
+    int persist_cache[1][17] = {};  // resident workgroups per CU per (kernel, block/64)
     int cus = 0;
     g4s_timing timing{};
+    g4s_launch_info linfo{};
+    // Per-step device time of the last run (ms), from the steps' start
+    // stamps on the device wall clock (clock_khz).
+    std::vector<float> step_ms;
+    int clock_khz = 0;
     std::vector<void *> allocs;
     // Event pairs bracketing each launch of the dominant kernel of a chunk
     // (g4s_run_opts.kernel_events); grown on demand, destroyed with the net.
@@ -145,18 +150,6 @@ hipError_t kev_reserve(g4s_net *net)
     return hipSuccess;
 }
 
-// Cooperative launches: record the opening (begin) or closing event of one
-// dominant-kernel launch on the stream.
-hipError_t kev_mark(g4s_net *net, bool begin)
-{
-    if (!net->kev_on) return hipSuccess;
-    if (begin) {
-        hipError_t r = kev_reserve(net);
-        if (r != hipSuccess) return r;
-    }
-    return hipEventRecord(net->kev[net->kev_used++], net->stream);
-}
-
 // Ordinary launches: hand the next event pair to the dispatch itself.
 hipError_t kev_stamp(g4s_net *net, Launch *L)
 {
@@ -185,6 +178,10 @@ hipError_t kev_collect(g4s_net *net, double *ms)
 int ensure_chunk(g4s_net *net, int chunk)
 {
     if (chunk <= net->chunk_cap) return G4S_OK;
+    // from here on the buffers are being replaced: a failed allocation below
+    // must leave a handle whose next call allocates again
+    net->chunk_cap = 0;
+    net->gen += 1;
     if (net->d.raster) { (void)hipFree(net->d.raster); net->allocs.erase(std::find(net->allocs.begin(), net->allocs.end(), (void *)net->d.raster)); }
     if (net->d.ctl) { (void)hipFree(net->d.ctl); net->allocs.erase(std::find(net->allocs.begin(), net->allocs.end(), (void *)net->d.ctl)); }
     for (int k = 0; k < 2; ++k) {
@@ -196,7 +193,7 @@ int ensure_chunk(g4s_net *net, int chunk)
     net->d.raster = nullptr; net->d.ctl = nullptr;
     int rc;
     if ((rc = net->alloc(&net->d.raster, (size_t)chunk * net->d.words))) return rc;
-    if ((rc = net->alloc(&net->d.ctl, (size_t)chunk * kCtlStride))) return rc;
+    if ((rc = net->alloc(&net->d.ctl, (size_t)(chunk + 1) * kCtlStride))) return rc;
     if (net->d.bcnt) {
         (void)hipFree(net->d.bcnt);
         net->allocs.erase(std::find(net->allocs.begin(), net->allocs.end(), (void *)net->d.bcnt));
@@ -205,32 +202,9 @@ int ensure_chunk(g4s_net *net, int chunk)
     if (net->d.P > 0 && (rc = net->alloc(&net->d.bcnt, (size_t)chunk * net->d.P))) return rc;
     for (int k = 0; k < 2; ++k) {
         HIP_OK(hipHostMalloc((void **)&net->h_raster[k], (size_t)chunk * net->d.words * sizeof(uint32_t)));
-        HIP_OK(hipHostMalloc((void **)&net->h_ctl[k], (size_t)chunk * kCtlStride * sizeof(int32_t)));
+        HIP_OK(hipHostMalloc((void **)&net->h_ctl[k], (size_t)(chunk + 1) * kCtlStride * sizeof(int32_t)));
     }
     net->chunk_cap = chunk;
-    net->gen += 1;
-    return G4S_OK;
-}
-
-int build_det_tables(g4s_net *net)
-{
-    if (net->det_ready) return G4S_OK;
-    const int64_t n = net->d.n;
-    const size_t ns = (size_t)n * (size_t)net->d.nsyn;
-    std::vector<uint2> tab(ns), in_tab(ns);
-    std::vector<int64_t> in_ptr((size_t)n + 1);
-    HIP_OK(hipMemcpy(tab.data(), net->d.syn, ns * sizeof(uint2), hipMemcpyDeviceToHost));
-    build_incoming(n, net->d.nsyn, tab.data(), in_ptr.data(), in_tab.data(), 0);
-    int64_t *dp = nullptr;
-    uint2 *dt = nullptr;
-    int rc;
-    if ((rc = net->alloc(&dp, (size_t)n + 1))) return rc;
-    if ((rc = net->alloc(&dt, ns))) return rc;
-    HIP_OK(hipMemcpy(dp, in_ptr.data(), ((size_t)n + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(dt, in_tab.data(), ns * sizeof(uint2), hipMemcpyHostToDevice));
-    net->d.in_ptr = dp;
-    net->d.in_syn = dt;
-    net->det_ready = true;
     net->gen += 1;
     return G4S_OK;
 }
@@ -242,18 +216,18 @@ Launch plain(g4s_net *net, int64_t threads, int block)
     return Launch{(int)g, block, net->stream};
 }
 
-// Grid for a cooperative kernel.  Kernels whose workgroups wait on each other
-// use at most half the co-resident capacity the occupancy API reports: the
-// API can overstate it (it does not see the SGPR limit of gfx9-family SIMDs,
-// 800 per SIMD, which caps kernels above 96 SGPRs at 7 waves), and a grid
-// that is not fully resident would wait forever.
-int coop_launch(g4s_net *net, CoopKernel k, int block, int64_t useful_threads, Launch *L)
+// Grid of a persistent kernel, whose workgroups wait on each other: at most
+// the workgroups the device holds at once (resident_per_cu: the compiler's
+// occupancy of the kernel, its LDS and the occupancy API, whichever is
+// least), so every workgroup is resident while the others wait for it.
+int persistent_launch(g4s_net *net, PersistentKernel k, int block, int64_t useful_threads, Launch *L)
 {
-    int &cached = net->coop_cache[(int)k][block / 64];
-    if (cached == 0) cached = coop_blocks_per_cu(k, block, 0);
+    int &cached = net->persist_cache[(int)k][block / 64];
+    if (cached == 0) cached = persistent_blocks_per_cu(k, block);
     if (net->cus == 0) net->cus = device_cus();
-    if (cached <= 0 || net->cus <= 0) return fail(G4S_ERR_HIP, "no co-resident capacity for cooperative kernel %d", (int)k);
-    const int per_cu = std::max(1, cached / 2);
+    if (cached <= 0 || net->cus <= 0)
+        return fail(G4S_ERR_HIP, "no resident capacity known for persistent kernel %d at block %d", (int)k, block);
+    const int per_cu = cached;
     const int maxb = per_cu * net->cus;
     int64_t want = (useful_threads + block - 1) / block;
     int g = (int)std::max<int64_t>(1, std::min<int64_t>(maxb, want));
@@ -286,6 +260,8 @@ int build_slices(g4s_net *net, int want)
     if (L > 16 * 1024)
         return fail(G4S_ERR_UNSUPPORTED, "N=%lld needs %lld slices of %lld: beyond the slice schemes' LDS", (long long)n,
                     (long long)P, (long long)L);
+    net->slices_ready = false;  // a failed allocation below leaves the handle to rebuild
+    net->gen += 1;
     for (void **pp : {(void **)&net->d.soff, (void **)&net->d.bcnt}) {
         if (*pp) {
             (void)hipFree(*pp);
@@ -329,20 +305,13 @@ void drop_graph(g4s_net *net)
     net->graph_steps = 0;
 }
 
-// AB chunk of `cs` steps: replayed from a captured HIP graph of cs
-// launches (captured again when the chunk length or any device pointer of
-// the network changed), or launched directly.
-int run_ab_chunk(g4s_net *net, int cs, bool use_graph, int block, bool det)
+// Graph of an AB chunk of `cs` steps: captured again when the chunk length,
+// tiling, mode or any device pointer of the network changed.  Called before
+// the chunk's opening event, so capture and instantiation (host work) stay
+// out of the chunk's device time.
+int prepare_ab_graph(g4s_net *net, int cs, int block, bool det)
 {
     const Launch L{net->d.P, block, net->stream};
-    if (!use_graph) {
-        for (int rel = 0; rel < cs; ++rel) {
-            Launch Lk = L;
-            HIP_OK(kev_stamp(net, &Lk));
-            HIP_OK(launch_slice_step(net->d, rel, det, Lk));
-        }
-        return G4S_OK;
-    }
     const int key = block * 2 + (det ? 1 : 0);
     if (!net->graph_exec || net->graph_steps != cs || net->graph_gen != net->gen || net->graph_block != key) {
         drop_graph(net);
@@ -361,7 +330,23 @@ int run_ab_chunk(g4s_net *net, int cs, bool use_graph, int block, bool det)
         net->graph_gen = net->gen;
         net->graph_block = key;
     }
-    HIP_OK(hipGraphLaunch(net->graph_exec, net->stream));
+    return G4S_OK;
+}
+
+// AB chunk of `cs` steps: replayed from the graph prepare_ab_graph made, or
+// launched directly.
+int run_ab_chunk(g4s_net *net, int cs, bool use_graph, int block, bool det)
+{
+    if (use_graph) {
+        HIP_OK(hipGraphLaunch(net->graph_exec, net->stream));
+        return G4S_OK;
+    }
+    const Launch L{net->d.P, block, net->stream};
+    for (int rel = 0; rel < cs; ++rel) {
+        Launch Lk = L;
+        HIP_OK(kev_stamp(net, &Lk));
+        HIP_OK(launch_slice_step(net->d, rel, det, Lk));
+    }
     return G4S_OK;
 }
 
@@ -410,6 +395,12 @@ int g4s_create(const g4s_spec *spec, g4s_net **out)
     auto bail = [&](int code) { g4s_destroy(net); return code; };
     if (hipStreamCreateWithFlags(&net->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(G4S_ERR_HIP, "hipStreamCreate failed"));
+    {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&net->clock_khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess)
+            net->clock_khz = 0;
+    }
     for (int k = 0; k < 2; ++k)
         if (hipEventCreate(&net->ev0[k]) != hipSuccess || hipEventCreate(&net->ev1[k]) != hipSuccess ||
             hipEventCreateWithFlags(&net->done_ev[k], hipEventDisableTiming) != hipSuccess)
@@ -464,41 +455,18 @@ bool uses_isyn(int alg, int accum)
     return accum == G4S_ACCUM_ATOMIC && (alg == G4S_ALG_AP || alg == G4S_ALG_N || alg == G4S_ALG_S);
 }
 
-// Launches of one step of a per-step scheme; returns the HIP status.
-hipError_t launch_step(g4s_net *net, int alg, int accum, int block, int t, int rel, int last, int *launches,
-                       Launch *L, int *rc)
+// Launches of one step of a per-step atomic scheme (AP, N, S); returns the
+// HIP status.  Det mode runs every scheme through the slice kernels.
+hipError_t launch_step(g4s_net *net, int alg, int block, int t, int rel, int last, int *launches, Launch *L, int *rc)
 {
     DevNet &d = net->d;
     *rc = G4S_OK;
     hipError_t e;
-    if (accum == G4S_ACCUM_DET) {
-        *launches += 1;
-        switch (alg) {
-        case G4S_ALG_AP:
-            if ((*rc = coop_launch(net, kCoopAPDet, block, d.n, L))) return hipSuccess;
-            if ((e = kev_mark(net, true)) != hipSuccess) return e;
-            e = launch_ap_det(d, t, rel, last, *L);
-            break;
-        case G4S_ALG_N:
-            *L = plain(net, d.n, block);
-            if ((e = kev_stamp(net, L)) != hipSuccess) return e;
-            return launch_pull_thread(d, t, rel, last, *L);
-        case G4S_ALG_S:
-            *L = plain(net, ((int64_t)d.n + 63) / 64 * 64, block);
-            if ((e = kev_stamp(net, L)) != hipSuccess) return e;
-            return launch_pull_wave(d, t, rel, last, *L);
-        default:  // AB and SKL run the slice kernels
-            *rc = fail(G4S_ERR_ARG, "scheme %d has no per-step det launch", alg);
-            return hipSuccess;
-        }
-        return e != hipSuccess ? e : kev_mark(net, false);
-    }
     if (alg == G4S_ALG_AP) {
         *launches += 1;
-        if ((*rc = coop_launch(net, kCoopAP, block, d.n, L))) return hipSuccess;
-        if ((e = kev_mark(net, true)) != hipSuccess) return e;
-        e = launch_ap(d, t, rel, last, net->last_count, *L);
-        return e != hipSuccess ? e : kev_mark(net, false);
+        if ((*rc = persistent_launch(net, kPersistAP, block, d.n, L))) return hipSuccess;
+        if ((e = kev_stamp(net, L)) != hipSuccess) return e;
+        return launch_ap(d, t, rel, last, net->last_count, *L);
     }
     *launches += 2;
     e = launch_update(d, t, rel, last, plain(net, d.n, block));
@@ -517,6 +485,38 @@ hipError_t launch_step(g4s_net *net, int alg, int accum, int block, int t, int r
 
 namespace {
 
+// Launch report of the dominant kernel of a run (g4s_get_launch_info).
+void record_launch(g4s_net *net, int alg, bool slice, int block, int grid, bool det)
+{
+    g4s_launch_info &li = net->linfo;
+    li = g4s_launch_info{};
+    if (net->cus == 0) net->cus = device_cus();
+    li.cus = net->cus;
+    li.grid = grid;
+    li.block = block;
+    char targs[32] = "";
+    const char *name;
+    if (slice) {
+        name = alg == G4S_ALG_SKL ? "k_skl_slice" : "k_slice_step";
+        slice_kernel_targs(net->d.L, block, det, targs, sizeof(targs));
+        li.lds_bytes = (int32_t)slice_lds(net->d.L, block, det);
+        li.resident_per_cu = alg == G4S_ALG_SKL ? skl_slice_blocks_per_cu(net->d.L, block, det)
+                                                : slice_step_blocks_per_cu(net->d.L, block, det);
+        li.persistent = alg == G4S_ALG_SKL;
+    } else {
+        name = alg == G4S_ALG_AP ? "k_ap" : alg == G4S_ALG_N ? "k_push_rows" : "k_push_flat";
+        li.persistent = alg == G4S_ALG_AP;
+        if (alg == G4S_ALG_AP) li.resident_per_cu = persistent_blocks_per_cu(kPersistAP, block);
+    }
+    snprintf(li.kernel, sizeof(li.kernel), targs[0] ? "%s<%s>" : "%s", name, targs);
+    if (const KernelRes *r = kernel_res(name, targs)) {
+        li.vgprs = r->vgprs;
+        li.sgprs = r->sgprs;
+        li.waves_per_simd = r->occupancy;
+        li.lds_bytes += r->lds;
+    }
+}
+
 int run_impl(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, int32_t *counts,
              const g4s_run_opts *opts, g4s_chunk_fn fn, void *user)
 {
@@ -532,11 +532,13 @@ int run_impl(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, 
     const int chunk = o.chunk > 0 ? o.chunk : 1000;
     int rc;
     if ((rc = ensure_chunk(net, chunk))) return rc;
-    // AB and SKL run the slice kernels in both modes; det N/S/AP gather
-    // from the incoming table.
-    const bool slice = alg == G4S_ALG_AB || alg == G4S_ALG_SKL;
+    // AB and SKL run the slice kernels in both modes.  Det mode runs every
+    // scheme through them: the ascending-source order of each input fixes a
+    // gather by target, which the slices' ordered accumulation is; AP, N and
+    // S then launch the slice step once per step like AB.
     const bool det = accum == G4S_ACCUM_DET;
-    if (det && !slice && (rc = build_det_tables(net))) return rc;
+    const bool slice = alg == G4S_ALG_AB || alg == G4S_ALG_SKL || det;
+    const bool slice_steps = slice && alg != G4S_ALG_SKL;  // per-step slice launches
     int sblock = 0;
     if (slice && o.slices > 0) {
         if ((rc = build_slices(net, o.slices))) return rc;
@@ -559,13 +561,14 @@ int run_impl(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, 
 
     DevNet &d = net->d;
     Launch L{};
-    if (alg == G4S_ALG_SKL && slice) {
-        // every workgroup must be resident: P slices within half the
-        // capacity reported for the kernel instantiation that will run
+    if (alg == G4S_ALG_SKL) {
+        // every workgroup must be resident: P slices within what the device
+        // holds at once of the kernel instantiation that will run
         const int per_cu = skl_slice_blocks_per_cu(d.L, sblock, det);
-        const int cap = std::max(1, per_cu / 2) * std::max(net->cus, 1);
+        const int cap = per_cu * std::max(net->cus, 1);
         if (per_cu <= 0 || cap < d.P)
-            return fail(G4S_ERR_UNSUPPORTED, "SKL needs %d co-resident workgroups, device holds %d", d.P, cap);
+            return fail(G4S_ERR_UNSUPPORTED, "SKL needs %d resident workgroups of %d threads, device holds %d", d.P,
+                        sblock, cap);
         L = Launch{d.P, sblock, net->stream};
     }
     // The global-atomic schemes read the pending input from isyn[t&1]; after
@@ -590,6 +593,7 @@ int run_impl(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, 
     // graph replay of AB steps for this run.
     net->kev_on = o.kernel_events != 0;
     net->kev_used = 0;
+    net->step_ms.assign((size_t)steps, 0.0f);
     const bool use_graph = o.graph >= 0 && !net->kev_on;
 
     const auto wall0 = std::chrono::steady_clock::now();
@@ -614,21 +618,22 @@ int run_impl(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, 
         HIP_OK(hipEventElapsedTime(&ms, net->ev0[pc.buf], net->ev1[pc.buf]));
         dev_ms += ms;
         if (net->kev_on) {
-            if (alg == G4S_ALG_SKL) {  // ev0..ev1 brackets exactly the one launch
-                kern_ms += ms;
-                kern_launches += 1;
-            } else {
-                kern_launches += net->kev_used / 2;
-                HIP_OK(kev_collect(net, &kern_ms));
-            }
+            kern_launches += net->kev_used / 2;
+            HIP_OK(kev_collect(net, &kern_ms));
         }
         const int32_t *hc = net->h_ctl[pc.buf];
         int64_t mx = 0;
+        auto stamp = [&](int r) {
+            return (uint64_t)(uint32_t)hc[r * kCtlStride + kCtlStamp] |
+                   ((uint64_t)(uint32_t)hc[r * kCtlStride + kCtlStamp + 1] << 32);
+        };
         for (int r = 0; r < pc.cs; ++r) {
             const int32_t cnt = hc[r * kCtlStride + kCtlCount];
             spikes += cnt;
             mx = std::max<int64_t>(mx, cnt);
             if (counts) counts[pc.c0 + r] = cnt;
+            if ((size_t)(pc.c0 + r) < net->step_ms.size() && net->clock_khz > 0)
+                net->step_ms[(size_t)(pc.c0 + r)] = (float)((double)(int64_t)(stamp(r + 1) - stamp(r)) / net->clock_khz);
         }
         net->last_count = hc[(pc.cs - 1) * kCtlStride + kCtlCount];
         net->est_count = std::max<int64_t>(64, mx + mx / 4);
@@ -652,43 +657,41 @@ int run_impl(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, 
         const int cs = (int)std::min<int64_t>(chunk, steps - c0);
         const int t0 = (int)(net->t);
         const int buf = chunk_no & 1;
-        HIP_OK(launch_fill_u32((uint32_t *)d.ctl, (size_t)cs * kCtlStride, 0u, net->stream));
+        if (slice_steps && use_graph && (rc = prepare_ab_graph(net, cs, sblock, det))) return abandon(rc);
+        HIP_OK(launch_fill_u32((uint32_t *)d.ctl, (size_t)(cs + 1) * kCtlStride, 0u, net->stream));
         HIP_OK(launch_fill_u32(d.bar, kBarWords, 0u, net->stream));
-        if (alg == G4S_ALG_AB && slice) {
+        if (slice_steps) {
             int32_t *ti = net->h_tinfo + 2 * buf;
             ti[0] = t0;
             ti[1] = (int32_t)(t_end - 1);
             HIP_OK(hipMemcpyAsync((void *)d.tinfo, ti, 2 * sizeof(int32_t), hipMemcpyHostToDevice, net->stream));
         }
         HIP_OK(hipEventRecord(net->ev0[buf], net->stream));
-        if (alg == G4S_ALG_AB && slice) {
+        if (slice_steps) {
             if ((rc = run_ab_chunk(net, cs, use_graph, sblock, det))) return abandon(rc);
             launches += cs;
             dom_grid = d.P;
         } else if (alg == G4S_ALG_SKL) {
-            hipError_t e;
-            e = launch_skl_slice(d, t0, cs, (int)(t_end - 1), det, L);
-            if (e == hipErrorIllegalState)
-                return abandon(fail(G4S_ERR_HIP, "SKL cooperative launch refused (%s); observed when another process "
-                                    "on this device has made cooperative launches", hipGetErrorString(e)));
-            if (e != hipSuccess) return abandon(fail(G4S_ERR_HIP, "SKL cooperative launch (grid %d): %s", L.grid, hipGetErrorString(e)));
+            Launch Lk = L;
+            HIP_OK(kev_stamp(net, &Lk));
+            const hipError_t e = launch_skl_slice(d, t0, cs, (int)(t_end - 1), det, Lk);
+            if (e != hipSuccess) return abandon(fail(G4S_ERR_HIP, "SKL launch (grid %d): %s", L.grid, hipGetErrorString(e)));
             launches += 1;
             dom_grid = L.grid;
         } else {
             for (int rel = 0; rel < cs; ++rel) {
                 const int t = t0 + rel;
-                hipError_t e = launch_step(net, alg, accum, block, t, rel, t == t_end - 1, &launches, &L, &rc);
+                hipError_t e = launch_step(net, alg, block, t, rel, t == t_end - 1, &launches, &L, &rc);
                 if (rc) return abandon(rc);
-                if (e == hipErrorIllegalState && alg == G4S_ALG_AP)
-                    return abandon(fail(G4S_ERR_HIP, "step %d: cooperative launch refused (%s); observed when another "
-                                        "process on this device has made cooperative launches", t, hipGetErrorString(e)));
                 if (e != hipSuccess) return abandon(fail(G4S_ERR_HIP, "step %d launch (alg %d, grid %d): %s", t, alg, L.grid, hipGetErrorString(e)));
                 dom_grid = L.grid;
             }
         }
         HIP_OK(hipEventRecord(net->ev1[buf], net->stream));
+        HIP_OK(launch_stamp(d.ctl, cs, net->stream));
         if (slice) HIP_OK(launch_reduce_counts(d, cs, net->stream));
-        HIP_OK(hipMemcpyAsync(net->h_ctl[buf], d.ctl, (size_t)cs * kCtlStride * sizeof(int32_t), hipMemcpyDeviceToHost, net->stream));
+        HIP_OK(hipMemcpyAsync(net->h_ctl[buf], d.ctl, (size_t)(cs + 1) * kCtlStride * sizeof(int32_t),
+                              hipMemcpyDeviceToHost, net->stream));
         if (raster || fn)
             HIP_OK(hipMemcpyAsync(net->h_raster[buf], d.raster, (size_t)cs * words * sizeof(uint32_t), hipMemcpyDeviceToHost, net->stream));
         HIP_OK(hipMemcpyAsync(&net->h_err[buf], d.err, sizeof(int32_t), hipMemcpyDeviceToHost, net->stream));
@@ -717,6 +720,7 @@ int run_impl(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, 
     net->timing.kernel_ms = kern_ms;
     net->timing.kernel_launches = kern_launches;
     net->kev_on = false;
+    record_launch(net, alg, slice, slice ? sblock : block, dom_grid, det);
     return G4S_OK;
 }
 
@@ -753,6 +757,35 @@ int g4s_get_timing(const g4s_net *net, g4s_timing *out)
 {
     if (!net || !out) return fail(G4S_ERR_ARG, "NULL argument");
     *out = net->timing;
+    return G4S_OK;
+}
+
+int g4s_get_step_times(const g4s_net *net, float *ms, int32_t n)
+{
+    if (!net || (!ms && n > 0) || n < 0) return fail(G4S_ERR_ARG, "bad arguments");
+    const size_t k = std::min<size_t>((size_t)n, net->step_ms.size());
+    if (k) memcpy(ms, net->step_ms.data(), k * sizeof(float));
+    return (int)k;
+}
+
+int g4s_get_launch_info(const g4s_net *net, g4s_launch_info *out)
+{
+    if (!net || !out) return fail(G4S_ERR_ARG, "NULL argument");
+    *out = net->linfo;
+    return G4S_OK;
+}
+
+int g4s_kernel_resources(const char *name, const char *targs, int32_t out[6])
+{
+    if (!name || !out) return fail(G4S_ERR_ARG, "NULL argument");
+    const KernelRes *r = kernel_res(name, targs ? targs : "");
+    if (!r) return fail(G4S_ERR_ARG, "no kernel %s<%s> in this build", name, targs ? targs : "");
+    out[0] = r->occupancy;
+    out[1] = r->vgprs;
+    out[2] = r->agprs;
+    out[3] = r->sgprs;
+    out[4] = r->lds;
+    out[5] = r->scratch;
     return G4S_OK;
 }
 

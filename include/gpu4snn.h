@@ -33,8 +33,8 @@ extern "C" {
 #define G4S_ALG_AP  0  /* per-spike work items (reference: child grid per spike, cu:408-409) */
 #define G4S_ALG_N   1  /* neuron-parallel (reference deliverspks1, cu:420-443)            */
 #define G4S_ALG_S   2  /* synapse-parallel (reference deliverspks2, cu:446-467)           */
-#define G4S_ALG_AB  3  /* persistent cooperative propagation (AB_deliverspks, cu:554-584)  */
-#define G4S_ALG_SKL 4  /* whole run in one cooperative launch (SKL_deliverspks, cu:669-716) */
+#define G4S_ALG_AB  3  /* slice-owned propagation, one launch per step (AB_deliverspks, cu:554-584) */
+#define G4S_ALG_SKL 4  /* a whole chunk of steps in one persistent launch (SKL_deliverspks, cu:669-716) */
 
 /* Accumulation modes of the synaptic input. */
 #define G4S_ACCUM_ATOMIC 0 /* fp32 atomic scatter, any order: the reference's semantics   */
@@ -115,6 +115,40 @@ int g4s_run_stream(g4s_net *net, int alg, int accum, int32_t steps, g4s_chunk_fn
                    const g4s_run_opts *opts);
 
 int g4s_get_timing(const g4s_net *net, g4s_timing *out);
+
+
+/* Device time of each step of the last g4s_run / g4s_run_stream, in ms:
+ * the interval between the starts of consecutive steps on the device's
+ * constant-rate wall clock (the last step of a chunk ends at the chunk's end
+ * stamp), i.e. kernel time plus the launch gaps inside the chunk.  Copies at
+ * most n values and returns how many (>= 0), or a G4S_ERR_* code.  The
+ * reference's per-step TimeKernelUpdate/TimeSpent rows of computetime.csv
+ * (cu:1219-1267). */
+int g4s_get_step_times(const g4s_net *net, float *ms, int32_t n);
+
+/* Launch report of the dominant kernel of the last g4s_run: the counterpart
+ * of the grid, block and occupancy figures the reference prints for each
+ * kernel it launches (cu:800-838, cu:1089-1201). */
+typedef struct g4s_launch_info {
+    char    kernel[64];       /* kernel and template arguments, e.g. "k_skl_slice<1,false,false>" */
+    int32_t grid, block;      /* workgroups and threads per workgroup                      */
+    int32_t lds_bytes;        /* dynamic + static LDS per workgroup                        */
+    int32_t vgprs, sgprs;     /* compiler's register allocation: per lane, per wave        */
+    int32_t waves_per_simd;   /* compiler's occupancy for those registers                  */
+    int32_t resident_per_cu;  /* workgroups of this launch one CU holds at once            */
+    int32_t cus;              /* compute units of the device                               */
+    int32_t persistent;       /* 1: its workgroups wait on each other, so the grid is at
+                                 most resident_per_cu * cus                               */
+} g4s_launch_info;
+int g4s_get_launch_info(const g4s_net *net, g4s_launch_info *out);
+
+/* Host-only: the compiler's resource report of one kernel instantiation of
+ * this build (name "k_skl_slice", template arguments "1,false,false"; "" for
+ * a kernel that is not a template).  out[0..5] = waves per SIMD, VGPRs,
+ * AGPRs, SGPRs, static LDS bytes per workgroup, scratch bytes per lane.
+ * G4S_ERR_ARG if the build has no such kernel.  No reference counterpart:
+ * the reference asks the occupancy API (cu:800-838). */
+int g4s_kernel_resources(const char *name, const char *targs, int32_t out[6]);
 
 /* Change the firing regime / thalamic scale of an existing network without
  * rebuilding its connectivity: the reference's per-run `mode` argument of

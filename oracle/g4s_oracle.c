@@ -321,6 +321,13 @@ int orc_simulate(const orc_net *net, int32_t t0, int32_t steps,
                  float *v, float *u, float *isyn, int32_t *nospks,
                  uint8_t *raster, int32_t *counts, float *lastI)
 {
+    return orc_simulate_order(net, t0, steps, v, u, isyn, nospks, raster, counts, lastI, 0);
+}
+
+int orc_simulate_order(const orc_net *net, int32_t t0, int32_t steps,
+                       float *v, float *u, float *isyn, int32_t *nospks,
+                       uint8_t *raster, int32_t *counts, float *lastI, int32_t descending)
+{
     const int64_t n = net->n;
     uint8_t *spk = (uint8_t *)malloc((size_t)n);
     if (!spk) return -2;
@@ -339,8 +346,10 @@ int orc_simulate(const orc_net *net, int32_t t0, int32_t steps,
             if (spk[i]) { nospks[i] += 1; cnt += 1; }
         }
         /* Propagation of step t into the input of step t+1, in ascending
-         * presynaptic order (cu:323-331 semantics without the atomics). */
-        for (int64_t i = 0; i < n; ++i) {
+         * presynaptic order (cu:323-331 semantics without the atomics), or
+         * descending for the reassociation study. */
+        for (int64_t k = 0; k < n; ++k) {
+            const int64_t i = descending ? n - 1 - k : k;
             if (!spk[i]) continue;
             const uint32_t *row = net->post + (size_t)i * (size_t)net->nsyn;
             const float *wr = net->w + (size_t)i * (size_t)net->nsyn;

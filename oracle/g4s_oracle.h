@@ -99,6 +99,17 @@ int orc_simulate(const orc_net *net, int32_t t0, int32_t steps,
                  float *v, float *u, float *isyn, int32_t *nospks,
                  uint8_t *raster, int32_t *counts, float *lastI);
 
+/* orc_simulate with the spiking rows of each step added in descending
+ * presynaptic order when `descending` is nonzero: the same terms summed in
+ * another order, i.e. one of the orders an atomic scatter may take.  Used
+ * to show that reassociation alone moves spike times.  Pinned: descending
+ * = 0 is orc_simulate bit for bit; with descending != 0 the per-step sums
+ * differ from the ascending ones only by rounding (same spikes for steps
+ * whose near-threshold margins exceed the rounding difference). */
+int orc_simulate_order(const orc_net *net, int32_t t0, int32_t steps,
+                       float *v, float *u, float *isyn, int32_t *nospks,
+                       uint8_t *raster, int32_t *counts, float *lastI, int32_t descending);
+
 /* orc_simulate on `threads` host threads (<= 0: one per core), bit-identical
  * to it: each thread updates a range of neurons, then sums into its own
  * range of targets the rows that spiked, in ascending presynaptic order

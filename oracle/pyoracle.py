@@ -66,6 +66,8 @@ def lib():
         L.orc_simulate.argtypes = [C.POINTER(OrcNet), C.c_int32, C.c_int32, _f32p, _f32p, _f32p,
                                    _i32p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_simulate.restype = C.c_int
+        L.orc_simulate_order.argtypes = list(L.orc_simulate.argtypes) + [C.c_int32]
+        L.orc_simulate_order.restype = C.c_int
         L.orc_simulate_f64_forced.argtypes = [C.POINTER(OrcNet), C.c_int32, C.c_int32, _u8p, _i64p,
                                               C.c_int64, _f64p, _f64p, C.c_void_p, C.c_void_p]
         L.orc_simulate_f64_forced.restype = C.c_int
@@ -167,14 +169,22 @@ class Network:
                       self.post.ctypes.data, self.w.ctypes.data, self.a.ctypes.data,
                       self.b.ctypes.data, self.c.ctypes.data, self.d.ctypes.data)
 
-    def run(self, steps, raster=True, threads=1):
+    def run(self, steps, raster=True, threads=1, descending=False):
         """Advance `steps` steps; returns (raster uint8[steps, n] or None, counts, lastI).
         threads != 1 runs the multithreaded form (same results bit for bit;
-        <= 0: one thread per core)."""
+        <= 0: one thread per core).  descending=True adds each step's
+        spiking rows in descending presynaptic order (single thread)."""
         r = np.zeros((steps, self.n), np.uint8) if raster else None
         counts = np.zeros(steps, np.int32)
         last = np.zeros(self.n, np.float32)
         cn = self._cnet()
+        if descending:
+            rc = lib().orc_simulate_order(C.byref(cn), self.t, steps, self.v, self.u, self.isyn, self.nospks,
+                                          r.ctypes.data if raster else None, counts.ctypes.data, last.ctypes.data, 1)
+            if rc != 0:
+                raise RuntimeError(f"orc_simulate_order failed ({rc})")
+            self.t += steps
+            return r, counts, last
         if threads != 1:
             rc = lib().orc_simulate_mt(C.byref(cn), self.t, steps, self.v, self.u, self.isyn, self.nospks,
                                        r.ctypes.data if raster else None, counts.ctypes.data, last.ctypes.data,

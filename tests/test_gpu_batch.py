@@ -212,3 +212,38 @@ def test_batch_tune_sets_bucket_plan(g4s, oracle):
         b.run()
         assert b.info(s)["slice_len"] == min(plan["slice_len"], 3520)
         check_instance(g4s, oracle, b, s, 3500, 300, 77, 1.0, 120)
+
+
+def test_batched_workload_at_its_size(g4s, oracle):
+    """The benchmark's batched workload itself: 256 networks, N log-uniform on
+    [1e3, 5e4], Nsyn in {100, 500, 1000}, thalamic x U(0.8, 1.5) (seed
+    20221), at the benchmark's chunk and default tiling, 200 steps.  Det: one
+    instance of every (floor(log2 N), Nsyn) bucket -- at least 16 -- matches
+    the oracle's per-step counts and final state bit for bit.  Atomic: the
+    whole batch's spike total is within 1% of det's."""
+    from bench import batch_specs
+    specs = batch_specs(256, 20221)
+    steps = 200
+    first = {}
+    for k, (n, ns, _, _) in enumerate(specs):
+        first.setdefault((n.bit_length() - 1, ns), k)
+    checked = sorted(first.values())
+    assert len(checked) >= 16, len(checked)
+    totals = {}
+    for accum in ("det", "atomic"):
+        with g4s.Batch(accum, chunk=100) as b:
+            slots = [b.add(n, ns, sd, steps, thal_scale=sc, keep_raster=False) for (n, ns, sd, sc) in specs]
+            b.run()
+            totals[accum] = sum(int(b.counts(s).sum()) for s in slots)
+            if accum == "det":
+                for k in checked:
+                    n, ns, sd, sc = specs[k]
+                    post, w = g4s.build_synapses_host(n, ns, sd)
+                    onet = oracle.Network(n, ns, sd, regime=1, thal_scale=sc, table=(post, w))
+                    _, ocounts, _ = onet.run(steps, raster=False)
+                    assert np.array_equal(b.counts(slots[k]), ocounts), (k, n, ns)
+                    v, u, _, nos = b.state(slots[k])
+                    assert np.array_equal(v.view(np.uint32), onet.v.view(np.uint32)), (k, n, ns)
+                    assert np.array_equal(u.view(np.uint32), onet.u.view(np.uint32)), (k, n, ns)
+                    assert np.array_equal(nos, onet.nospks), (k, n, ns)
+    assert abs(totals["atomic"] - totals["det"]) <= 0.01 * totals["det"], totals

@@ -56,21 +56,31 @@ def test_n1e6_fp32_voltage_error(g4s, big_net, f64_oracle, alg):
     assert p99 < 1.0
 
 
-@pytest.mark.parametrize("alg", ["AB", "SKL"])
-def test_n1e6_det_bitwise(g4s, oracle, big_net, alg):
-    """Det mode at N=10^6 (255 slices of 3936 neurons, 1024 threads: the large
-    plan, about 16k spikes per step in two passes) is bit-identical to the fp32
-    oracle run on the same synapse table for the first 20 steps."""
-    steps = 20
+DET_STEPS = 100
+
+
+@pytest.fixture(scope="module")
+def det_oracle(oracle, big_net):
+    """The fp32 oracle's first DET_STEPS steps on the device's synapse table."""
     post, w = big_net.synapses()
     onet = oracle.Network(N, NSYN, SEED, regime=REGIME, table=(post, w))
-    oraster, ocounts, _ = onet.run(steps, threads=16)
-    del post, w
+    oraster, ocounts, _ = onet.run(DET_STEPS, threads=16)
+    v, u = onet.v.copy(), onet.u.copy()
+    del onet, post, w
+    return oraster, ocounts, v, u
+
+
+@pytest.mark.parametrize("alg", ["AB", "SKL"])
+def test_n1e6_det_bitwise(g4s, big_net, det_oracle, alg):
+    """Det mode at N=10^6 (255 slices of 3936 neurons, 1024 threads: the large
+    plan, about 16k spikes per step in two passes) is bit-identical to the fp32
+    oracle run on the same synapse table for the first 100 steps."""
+    oraster, ocounts, ov, ou = det_oracle
     big_net.reset()
-    words, counts = big_net.run(steps, alg=alg, accum="det", slices=256, block=1024)
+    words, counts = big_net.run(DET_STEPS, alg=alg, accum="det", slices=256, block=1024)
     assert big_net.timing()["grid"] == 255
     assert np.array_equal(counts, ocounts), alg
     assert np.array_equal(g4s.raster_to_dense(words, N), oraster), alg
     v, u, _, _ = big_net.state()
-    assert np.array_equal(v.view(np.uint32), onet.v.view(np.uint32)), alg
-    assert np.array_equal(u.view(np.uint32), onet.u.view(np.uint32)), alg
+    assert np.array_equal(v.view(np.uint32), ov.view(np.uint32)), alg
+    assert np.array_equal(u.view(np.uint32), ou.view(np.uint32)), alg

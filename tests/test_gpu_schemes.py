@@ -116,44 +116,54 @@ def test_quiet_regime_and_zero_steps(g4s, oracle):
 
 @pytest.fixture(scope="module")
 def big():
-    """N=100,000, Nsyn=1000, balanced regime."""
+    """N=100,000, Nsyn=1000, balanced regime: the headline network."""
     return dict(n=100_000, nsyn=1000, seed=20221)
 
 
+BIG_STEPS = 1000
+
+
 @pytest.fixture(scope="module")
-def big_oracle(oracle, big):
-    """The oracle's first 60 steps of the N=100,000 network."""
-    return oracle_raster(oracle, big["n"], big["nsyn"], 60, big["seed"])
+def big_oracle(g4s, oracle, big):
+    """The oracle's whole 1000-step run of the N=100,000 network (threaded
+    form, bit-identical to the single-thread oracle), on the host builder's
+    table (equal to the oracle's own builder; checked against the device
+    table below)."""
+    post, w = g4s.build_synapses_host(big["n"], big["nsyn"], big["seed"])
+    onet = oracle.Network(big["n"], big["nsyn"], big["seed"], table=(post, w))
+    r, c, _ = onet.run(BIG_STEPS, threads=16)
+    return onet, r, c
 
 
 def test_n100k_det_all_schemes_identical_and_oracle(g4s, big, big_oracle):
+    """Every scheme's det run of the whole configuration (1000 steps) is
+    bit-identical to the oracle: raster, counts, v, u, spike counts."""
     n, nsyn, seed = big["n"], big["nsyn"], big["seed"]
-    steps = 60
-    onet, oraster, _ = big_oracle
+    onet, oraster, ocounts = big_oracle
     with g4s.Network(n, nsyn, seed) as net:
         post, w = net.synapses()
         assert np.array_equal(post, onet.post) and np.array_equal(w.view(np.uint32), onet.w.view(np.uint32))
-        ref = None
+        del post, w
         for alg in ALGS:
             net.reset()
-            words, counts = net.run(steps, alg=alg, accum="det")
-            d = dense(g4s, words, n)
-            assert np.array_equal(d, oraster), alg
-            if ref is None:
-                ref = words
-            assert np.array_equal(words, ref), alg
-        v, u, _, _ = net.state()
-        assert np.array_equal(v.view(np.uint32), onet.v.view(np.uint32))
+            words, counts = net.run(BIG_STEPS, alg=alg, accum="det")
+            assert np.array_equal(counts, ocounts), (alg, int(np.argmax(counts != ocounts)))
+            assert np.array_equal(dense(g4s, words, n), oraster), alg
+            v, u, _, nos = net.state()
+            assert np.array_equal(v.view(np.uint32), onet.v.view(np.uint32)), alg
+            assert np.array_equal(u.view(np.uint32), onet.u.view(np.uint32)), alg
+            assert np.array_equal(nos, onet.nospks), alg
 
 
 @pytest.mark.parametrize("alg", ["AB", "SKL"])
 def test_n100k_bench_plan_det_bitwise(g4s, big, big_oracle, alg):
     """The tiling the bench times (256 slices requested, 1024 threads per
-    workgroup) is bit-identical to the oracle in det mode."""
+    workgroup) is bit-identical to the oracle in det mode over the whole
+    1000-step configuration."""
     n, nsyn, seed = big["n"], big["nsyn"], big["seed"]
     onet, oraster, _ = big_oracle
     with g4s.Network(n, nsyn, seed) as net:
-        words, _ = net.run(60, alg=alg, accum="det", slices=256, block=1024)
+        words, _ = net.run(BIG_STEPS, alg=alg, accum="det", slices=256, block=1024)
         assert net.timing()["grid"] == 241  # slices of 416 neurons
         assert np.array_equal(dense(g4s, words, n), oraster)
         v, u, _, _ = net.state()
@@ -161,15 +171,79 @@ def test_n100k_bench_plan_det_bitwise(g4s, big, big_oracle, alg):
         assert np.array_equal(u.view(np.uint32), onet.u.view(np.uint32))
 
 
-def test_n100k_atomic_rates(g4s, big):
+# Atomic mode at N=100,000 leaves the oracle's raster once a reordered
+# binary32 addition moves a near-threshold neuron across 30 mV.  Over five
+# seeds and every scheme (three runs each) that happened after 36 to 167
+# steps, at the same steps at which a sequential CPU run in descending
+# presynaptic order leaves it (test_oracle_model.py); for this seed after
+# 48, 58 or 114 steps.  The bar asserted here is the lower bound that holds
+# for this seed, with margin; 100 steps holds at N=2500 (test_gpu_parity).
+N100K_ATOMIC_IDENTICAL_STEPS = 40
+
+
+@pytest.mark.parametrize("plan", ["default", "bench"])
+def test_n100k_atomic_identity_and_rate_vs_oracle(g4s, big, big_oracle, plan):
+    """Each scheme in atomic mode: the oracle's raster for the first
+    N100K_ATOMIC_IDENTICAL_STEPS steps, and its 1000-step spike total within
+    1% of the oracle's."""
     n, nsyn, seed = big["n"], big["nsyn"], big["seed"]
+    _, oraster, ocounts = big_oracle
+    ref = int(ocounts.sum())
     with g4s.Network(n, nsyn, seed) as net:
-        _, cdet = net.run(1000, alg="SKL", accum="det")
-        ref = cdet.sum()
+        for alg in ALGS:
+            if plan == "bench" and alg not in ("AB", "SKL"):
+                continue
+            kw = dict(slices=256, block=1024) if plan == "bench" else {}
+            net.reset()
+            words, c = net.run(BIG_STEPS, alg=alg, accum="atomic", **kw)
+            k = N100K_ATOMIC_IDENTICAL_STEPS
+            d = dense(g4s, words[:k], n)
+            assert np.array_equal(d, oraster[:k]), (alg, plan, int(np.argmax(np.any(d != oraster[:k], axis=1))))
+            assert abs(int(c.sum()) - ref) <= 0.01 * ref, (alg, plan, int(c.sum()), ref)
+
+
+@pytest.mark.parametrize("accum", ["det", "atomic"])
+def test_n100k_skl_two_slices_per_cu(g4s, big, big_oracle, accum):
+    """SKL's grid is sized from the compiler's occupancy of the kernel it
+    launches: at 512 threads the slice kernel holds two workgroups per CU, so
+    447 slices of 224 neurons (more than the 256 CUs) are all resident and
+    the grid barrier completes.  Det: the oracle bit for bit over the whole
+    configuration; atomic: the single-network atomic bar."""
+    n, nsyn, seed = big["n"], big["nsyn"], big["seed"]
+    onet, oraster, ocounts = big_oracle
+    with g4s.Network(n, nsyn, seed) as net:
+        words, c = net.run(BIG_STEPS, alg="SKL", accum=accum, slices=512, block=512)
+        li = net.launch_info()
+        assert li["grid"] == 447 and li["persistent"] == 1, li
+        assert li["resident_per_cu"] >= 2 and li["grid"] <= li["resident_per_cu"] * li["cus"], li
+        assert li["grid"] > li["cus"], li
+        if accum == "det":
+            assert np.array_equal(dense(g4s, words, n), oraster)
+            v, _, _, _ = net.state()
+            assert np.array_equal(v.view(np.uint32), onet.v.view(np.uint32))
+        else:
+            k = N100K_ATOMIC_IDENTICAL_STEPS
+            assert np.array_equal(dense(g4s, words[:k], n), oraster[:k])
+            assert abs(int(c.sum()) - int(ocounts.sum())) <= 0.01 * int(ocounts.sum())
+
+
+def test_step_times_and_launch_info(g4s):
+    """Per-step device times (the reference's computetime.csv rows) sum to
+    the run's device time, and the launch report names the kernel run."""
+    with g4s.Network(20000, 500, 3) as net:
         for alg in ALGS:
             net.reset()
-            _, c = net.run(1000, alg=alg, accum="atomic")
-            assert abs(c.sum() - ref) <= 0.01 * ref, (alg, c.sum(), ref)
+            net.run(300, alg=alg, accum="atomic", chunk=128)
+            st = net.step_times()
+            tm = net.timing()
+            assert st.shape == (300,) and (st > 0).all(), alg
+            # stamps start at each step's first kernel: the chunk's sum is
+            # within a few microseconds per chunk of the event-timed total
+            assert abs(float(st.sum()) - tm["device_ms"]) <= 0.05 * tm["device_ms"] + 0.05, (alg, st.sum(), tm)
+            li = net.launch_info()
+            want = {"AP": "k_ap", "N": "k_push_rows", "S": "k_push_flat", "AB": "k_slice_step<",
+                    "SKL": "k_skl_slice<"}[alg]
+            assert li["kernel"].startswith(want) and li["waves_per_simd"] >= 1, (alg, li)
 
 
 @pytest.mark.parametrize("alg", ALGS)

@@ -76,15 +76,35 @@ def test_cli_result_files_det(g4s, oracle, tmp_path):
     assert after.shape == (n, 9) and np.array_equal(after[:, 8].astype(np.int64), onet.nospks)
 
 
-def test_driver_layout(g4s, tmp_path):
-    """The driver in a child process.  It runs the non-cooperative schemes
-    here: this test process has already made cooperative launches (AP, SKL),
-    and the runtime refuses them to a second process on the same device."""
-    p = subprocess.run([sys.executable, os.path.join(ROOT, "gpu4snn.py"), "MI355X", "120", "--alg", "N"],
+def test_driver_default_sweep(g4s, tmp_path):
+    """The drop-in driver with no flags, in a child process: the reference's
+    default run (py:42, cu:1082-1083), 3 regimes x 5 schemes -- AP and SKL
+    included, whose persistent grids are ordinary launches sized to the
+    device's residency -- laid out as GPU_NAME/ITERATIONS/Results with the
+    five files of every run."""
+    steps = 200
+    p = subprocess.run([sys.executable, os.path.join(ROOT, "gpu4snn.py"), "MI355X", str(steps)],
                        cwd=tmp_path, capture_output=True, text=True, timeout=300)
     assert p.returncode == 0, p.stderr[-2000:]
-    base = tmp_path / "MI355X" / "120" / "Results"
+    assert p.stdout.count("Launched ") == 15
+    base = tmp_path / "MI355X" / str(steps) / "Results"
     runs = sorted(x.name for x in base.iterdir())
-    assert runs == [f"N2500Nsyn1000Regime{r}Alg1" for r in range(3)]
+    assert runs == sorted(f"N2500Nsyn1000Regime{r}Alg{a}" for r in range(3) for a in range(5))
+    totals = {}
     for r in runs:
-        assert len(list((base / r).iterdir())) == 5
+        d = base / r
+        assert sorted(x.name for x in d.iterdir()) == sorted(
+            ["neuroninfo.csv", "spiketimes.csv", "computetime.csv", "sim_overview.csv", "neuroninfoafter.csv"])
+        ct = np.loadtxt(d / "computetime.csv", skiprows=1)
+        assert ct.shape == (steps, 4)
+        # per-step device times, not one average repeated
+        assert (ct[:, 2] > 0).all() and (ct[:, 3] >= ct[:, 2]).all() and np.unique(ct[:, 2]).size > 1
+        ov = (d / "sim_overview.csv").read_text()
+        totals[r] = int(ov.split("\n")[1].split()[3])
+        assert totals[r] == int(ct[:, 1].sum())
+    for regime in (1, 2):  # the balanced and irregular runs fire under every scheme
+        for a in range(5):
+            name = f"N2500Nsyn1000Regime{regime}Alg{a}"
+            assert totals[name] > 0, name
+            spk = (base / name / "spiketimes.csv").read_text().splitlines()
+            assert len(spk) == totals[name] + 1

@@ -239,3 +239,41 @@ def test_multithreaded_oracle_is_bit_identical(oracle, threads):
     assert np.array_equal(a.u.view(np.uint32), b.u.view(np.uint32))
     assert np.array_equal(la.view(np.uint32), lb.view(np.uint32))
     assert np.array_equal(a.nospks, b.nospks)
+
+
+def test_descending_order_is_a_reassociation(oracle):
+    """descending=False is the oracle itself; descending=True sums the same
+    terms in another order: on a small network the two runs keep the same
+    spikes and their states differ only by rounding over a short horizon."""
+    n, nsyn, seed, steps = 400, 100, 3, 40
+    a = oracle.Network(n, nsyn, seed, regime=2)
+    b = oracle.Network(n, nsyn, seed, regime=2)
+    c = oracle.Network(n, nsyn, seed, regime=2)
+    ra, _, _ = a.run(steps)
+    rb, _, _ = b.run(steps, descending=True)
+    rc, _, _ = c.run(steps, threads=1)
+    assert np.array_equal(ra, rc) and np.array_equal(a.v.view(np.uint32), c.v.view(np.uint32))
+    assert np.array_equal(ra, rb)
+    assert np.abs(a.v - b.v).max() < 1e-2 and not np.array_equal(a.v, b.v)
+
+
+@pytest.mark.parametrize("n,seed,first", [(100_000, 20221, 58), (100_000, 1, 50)])
+def test_reassociation_alone_moves_spikes_at_n100k(g4s, oracle, n, seed, first):
+    """Why atomic mode cannot keep the deterministic raster for 100 steps at
+    N=100,000: a sequential CPU run that adds each step's rows in descending
+    instead of ascending presynaptic order -- no GPU, no atomics, the same
+    terms -- leaves the ascending raster at step 58 (seed 20221) or 50
+    (seed 1).  A one-ulp difference in one input grows by about 1.5x per
+    step through the network's dynamics (7.6e-6 mV at step 4, 1 mV by step
+    32), and with 1e5 neurons some neuron lands within ~1e-2 mV of the 30 mV
+    threshold in almost every step.  The GPU's atomic runs leave the
+    raster at the same steps (48/58/114 and 36-74 over repeated runs)."""
+    post, w = g4s.build_synapses_host(n, 1000, seed)
+    asc = oracle.Network(n, 1000, seed, table=(post, w))
+    desc = oracle.Network(n, 1000, seed, table=(post, w))
+    del post, w
+    steps = first + 2
+    ra, _, _ = asc.run(steps, threads=0)
+    rd, _, _ = desc.run(steps, descending=True)
+    diff = np.nonzero(np.any(ra != rd, axis=1))[0]
+    assert diff.size and int(diff[0]) == first, diff[:5]
