@@ -117,6 +117,7 @@ def lib():
             "g4s_get_launch_info": ([_vp, C.POINTER(LaunchInfo)], C.c_int),
             "g4s_kernel_resources": ([C.c_char_p, C.c_char_p, _vp], C.c_int),
             "g4s_run_stream": ([_vp, C.c_int, C.c_int, C.c_int32, CHUNK_FN, _vp, C.POINTER(RunOpts)], C.c_int),
+            "g4s_run_noalloc": ([_vp, C.c_int, C.c_int, C.c_int32, _vp, _vp, C.POINTER(RunOpts)], C.c_int),
             "g4s_current_step": ([_vp], C.c_int64),
             "g4s_get_state": ([_vp, _vp, _vp, _vp, _vp], C.c_int),
             "g4s_get_params": ([_vp, _vp, _vp, _vp, _vp], C.c_int),

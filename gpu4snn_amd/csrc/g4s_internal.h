@@ -93,6 +93,8 @@ struct Launch {
 // Kernel launchers (g4s_kernels.hip).  Each returns the HIP status.
 hipError_t launch_init(const DevNet &d, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, size_t n, uint32_t val, hipStream_t s);
+// counts[r] = spike count of control row r, r < steps.
+hipError_t launch_copy_counts(const int32_t *ctl, int32_t *counts, int steps, hipStream_t s);
 // Stamp the end of a chunk of `rel` steps into control row `rel`.
 hipError_t launch_stamp(int32_t *ctl, int rel, hipStream_t s);
 hipError_t launch_update(const DevNet &d, int t, int rel, int last, const Launch &L);

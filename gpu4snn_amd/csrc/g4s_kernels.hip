@@ -276,6 +276,13 @@ __global__ void __launch_bounds__(256) k_fill_u32(uint32_t *p, size_t n, uint32_
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = val;
 }
 
+// counts[r] = spikes of chunk step r, r < steps (from the control rows).
+__global__ void __launch_bounds__(256) k_copy_counts(const int32_t *ctl, int32_t *counts, int steps)
+{
+    for (int r = (int)(blockIdx.x * blockDim.x + threadIdx.x); r < steps; r += (int)(gridDim.x * blockDim.x))
+        counts[r] = ctl[r * kCtlStride + kCtlCount];
+}
+
 // The chunk's end stamp (control row `rel` = the chunk's length).
 __global__ void __launch_bounds__(64) k_stamp(int32_t *ctl, int rel)
 {
@@ -1580,6 +1587,13 @@ hipError_t launch_init(const DevNet &d, hipStream_t s)
 {
     const int64_t m = d.n > d.words ? d.n : d.words;
     hipLaunchKernelGGL(k_init, dim3(grid_for(m, 256)), dim3(256), 0, s, d);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_counts(const int32_t *ctl, int32_t *counts, int steps, hipStream_t s)
+{
+    if (steps <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_copy_counts, dim3(grid_for(steps, 256)), dim3(256), 0, s, ctl, counts, steps);
     return hipGetLastError();
 }
 

@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -22,6 +23,57 @@ namespace {
 thread_local std::string g_err;
 }  // namespace
 
+// Device scratch of g4s_run_noalloc: the per-step control rows, per-slice
+// spike counts, grid-barrier counters, error flag and step info that
+// g4s_run keeps in buffers of the handle.  One array per device for the
+// life of the process; calls on one device share it (see
+// g4s_run_noalloc).  Host code uses the address returned by
+// hipGetSymbolAddress for the current device, never the array's name.
+static __device__ __align__(256) unsigned char scratch[G4S_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device.
+int scratch_of_current_device(unsigned char **out)
+{
+    int dev = 0;
+    G4S_HIP_OK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(G4S_ERR_UNSUPPORTED, "device %d beyond %d", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(G4S_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+        g_scratch[dev] = (unsigned char *)p;
+    }
+    *out = g_scratch[dev];
+    return G4S_OK;
+}
+
+// Layout of the scratch array for a run of chunks of `cs` steps over P
+// slices: barrier counters, error flag, step info, control rows (cs + 1),
+// per-slice counts (cs x P).
+struct ScratchLayout {
+    size_t bar, err, tinfo, ctl, bcnt, end;
+};
+ScratchLayout scratch_layout(int cs, int P)
+{
+    ScratchLayout l{};
+    auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+    l.bar = 0;
+    l.err = up(l.bar + (size_t)kBarWords * 4);
+    l.tinfo = up(l.err + 4);
+    l.ctl = up(l.tinfo + 8);
+    l.bcnt = up(l.ctl + (size_t)(cs + 1) * kCtlStride * 4);
+    l.end = l.bcnt + (size_t)cs * (size_t)std::max(P, 0) * 4;
+    return l;
+}
+}  // namespace
 
 namespace g4s {
 
@@ -517,8 +569,34 @@ void record_launch(g4s_net *net, int alg, bool slice, int block, int grid, bool 
     }
 }
 
+// Outputs of g4s_run_noalloc: device buffers of the caller.
+struct NoAllocOut {
+    uint32_t *raster;  // steps x words, or NULL
+    int32_t *counts;   // steps, or NULL
+};
+
+// Restores the handle's own buffers after a g4s_run_noalloc call, whatever
+// path the call leaves by.
+struct ScratchSwap {
+    g4s_net *net = nullptr;
+    int32_t *ctl = nullptr, *bcnt = nullptr, *err = nullptr;
+    uint32_t *bar = nullptr, *raster = nullptr;
+    const int32_t *tinfo = nullptr;
+    void take(g4s_net *n)
+    {
+        net = n;
+        ctl = n->d.ctl; bcnt = n->d.bcnt; err = n->d.err; bar = n->d.bar; raster = n->d.raster; tinfo = n->d.tinfo;
+    }
+    ~ScratchSwap()
+    {
+        if (!net) return;
+        net->d.ctl = ctl; net->d.bcnt = bcnt; net->d.err = err; net->d.bar = bar; net->d.raster = raster;
+        net->d.tinfo = tinfo;
+    }
+};
+
 int run_impl(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, int32_t *counts,
-             const g4s_run_opts *opts, g4s_chunk_fn fn, void *user)
+             const g4s_run_opts *opts, g4s_chunk_fn fn, void *user, const NoAllocOut *na = nullptr)
 {
     if (!net) return fail(G4S_ERR_ARG, "net is NULL");
     if (alg < G4S_ALG_AP || alg > G4S_ALG_SKL) return fail(G4S_ERR_ARG, "unknown alg %d", alg);
@@ -529,9 +607,17 @@ int run_impl(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, 
     if (opts) o = *opts;
     if (o.block < 0 || o.block % 64 != 0 || o.block > 1024) return fail(G4S_ERR_ARG, "block must be a multiple of 64 <= 1024");
     const int block = o.block > 0 ? std::min(o.block, 256) : 256;  // launch bound of the other kernels
-    const int chunk = o.chunk > 0 ? o.chunk : 1000;
+    int chunk = o.chunk > 0 ? o.chunk : 1000;
     int rc;
-    if ((rc = ensure_chunk(net, chunk))) return rc;
+    if (na) {
+        // nothing is allocated: the handle's staging (chunk_cap rows) and
+        // slice partition must already exist
+        if (o.kernel_events) return fail(G4S_ERR_ARG, "kernel_events needs event allocation: not in g4s_run_noalloc");
+        if (net->chunk_cap < 1) return fail(G4S_ERR_STATE, "handle has no staging buffers");
+        chunk = std::min(chunk, net->chunk_cap);
+    } else if ((rc = ensure_chunk(net, chunk))) {
+        return rc;
+    }
     // AB and SKL run the slice kernels in both modes.  Det mode runs every
     // scheme through them: the ascending-source order of each input fixes a
     // gather by target, which the slices' ordered accumulation is; AP, N and
@@ -540,7 +626,13 @@ int run_impl(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, 
     const bool slice = alg == G4S_ALG_AB || alg == G4S_ALG_SKL || det;
     const bool slice_steps = slice && alg != G4S_ALG_SKL;  // per-step slice launches
     int sblock = 0;
-    if (slice && o.slices > 0) {
+    if (slice && na) {
+        // the partition the handle holds (that of its last g4s_run of a
+        // slice scheme); building one would allocate
+        if (!net->slices_ready)
+            return fail(G4S_ERR_STATE, "no slice partition: run g4s_run once with this scheme and options first");
+        sblock = slice_block(net, o.block);
+    } else if (slice && o.slices > 0) {
         if ((rc = build_slices(net, o.slices))) return rc;
         sblock = slice_block(net, o.block);
     } else if (slice) {
@@ -593,8 +685,36 @@ int run_impl(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, 
     // graph replay of AB steps for this run.
     net->kev_on = o.kernel_events != 0;
     net->kev_used = 0;
-    net->step_ms.assign((size_t)steps, 0.0f);
-    const bool use_graph = o.graph >= 0 && !net->kev_on;
+    if (na)
+        net->step_ms.clear();  // keeps its capacity: no host allocation either
+    else
+        net->step_ms.assign((size_t)steps, 0.0f);
+    const bool use_graph = o.graph >= 0 && !net->kev_on && !na;
+    // g4s_run_noalloc: the run's scratch buffers from the device's static
+    // array, chunks no longer than it holds; the handle's own buffers are
+    // restored on return
+    ScratchSwap swap;
+    if (na) {
+        unsigned char *sc = nullptr;
+        if ((rc = scratch_of_current_device(&sc))) return rc;
+        const int P = slice ? d.P : 0;
+        int cs_max = chunk;
+        while (cs_max > 0 && scratch_layout(cs_max, P).end > (size_t)G4S_NOALLOC_SCRATCH_BYTES) cs_max /= 2;
+        if (cs_max < 1)
+            return fail(G4S_ERR_UNSUPPORTED, "%d slices need %zu bytes of scratch per step; the static scratch holds %d",
+                        P, scratch_layout(1, P).end, G4S_NOALLOC_SCRATCH_BYTES);
+        chunk = cs_max;
+        const ScratchLayout l = scratch_layout(chunk, P);
+        swap.take(net);
+        d.bar = (uint32_t *)(sc + l.bar);
+        d.err = (int32_t *)(sc + l.err);
+        d.tinfo = (const int32_t *)(sc + l.tinfo);
+        d.ctl = (int32_t *)(sc + l.ctl);
+        d.bcnt = slice ? (int32_t *)(sc + l.bcnt) : nullptr;
+        d.raster = nullptr;
+        // the array keeps the last call's contents: the error flag starts clear
+        HIP_OK(launch_fill_u32((uint32_t *)d.err, 1, 0u, net->stream));
+    }
 
     const auto wall0 = std::chrono::steady_clock::now();
     double dev_ms = 0.0, kern_ms = 0.0;
@@ -660,6 +780,7 @@ int run_impl(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, 
         if (slice_steps && use_graph && (rc = prepare_ab_graph(net, cs, sblock, det))) return abandon(rc);
         HIP_OK(launch_fill_u32((uint32_t *)d.ctl, (size_t)(cs + 1) * kCtlStride, 0u, net->stream));
         HIP_OK(launch_fill_u32(d.bar, kBarWords, 0u, net->stream));
+        if (na) d.raster = na->raster ? na->raster + (size_t)c0 * (size_t)words : nullptr;
         if (slice_steps) {
             int32_t *ti = net->h_tinfo + 2 * buf;
             ti[0] = t0;
@@ -690,6 +811,7 @@ int run_impl(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, 
         HIP_OK(hipEventRecord(net->ev1[buf], net->stream));
         HIP_OK(launch_stamp(d.ctl, cs, net->stream));
         if (slice) HIP_OK(launch_reduce_counts(d, cs, net->stream));
+        if (na && na->counts) HIP_OK(launch_copy_counts(d.ctl, na->counts + c0, cs, net->stream));
         HIP_OK(hipMemcpyAsync(net->h_ctl[buf], d.ctl, (size_t)(cs + 1) * kCtlStride * sizeof(int32_t),
                               hipMemcpyDeviceToHost, net->stream));
         if (raster || fn)
@@ -730,6 +852,13 @@ int g4s_run(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, i
             const g4s_run_opts *opts)
 {
     return run_impl(net, alg, accum, steps, raster, counts, opts, nullptr, nullptr);
+}
+
+int g4s_run_noalloc(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster_dev, int32_t *counts_dev,
+                    const g4s_run_opts *opts)
+{
+    const NoAllocOut na{raster_dev, counts_dev};
+    return run_impl(net, alg, accum, steps, nullptr, nullptr, opts, nullptr, nullptr, &na);
 }
 
 int g4s_run_stream(g4s_net *net, int alg, int accum, int32_t steps, g4s_chunk_fn fn, void *user,

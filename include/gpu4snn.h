@@ -116,6 +116,29 @@ int g4s_run_stream(g4s_net *net, int alg, int accum, int32_t steps, g4s_chunk_fn
 
 int g4s_get_timing(const g4s_net *net, g4s_timing *out);
 
+/* No counterpart in the reference.  For callers that cannot allocate
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  g4s_run on a handle that a g4s_run of the same scheme
+ * has prepared (its slice partition and staging exist; a zero-step g4s_run
+ * with the same options prepares it), writing the raster rows (layout as
+ * g4s_run's, steps x ceil(N/32) words) and the per-step spike counts into
+ * the caller's DEVICE buffers raster_dev and counts_dev (either may be
+ * NULL).  It calls no hipMalloc, hipFree, hipHostMalloc, event or graph
+ * creation: every buffer the run needs beyond the network's own state (the
+ * per-step control rows, per-slice spike counts, grid-barrier counters, the
+ * error flag and step info) comes from one static device array of
+ * G4S_NOALLOC_SCRATCH_BYTES per device, held until the process ends.  Runs
+ * are cut into chunks that fit it; a network whose slices need more than it
+ * holds for one step (P > about 260000 slices) gets G4S_ERR_UNSUPPORTED.
+ * It uses the slice partition the handle holds (opts->slices is not
+ * applied) and launches AB steps directly (no graph); kernel_events is
+ * refused (G4S_ERR_ARG).  Every call on one device shares that device's
+ * array, which keeps the last call's contents: calls that overlap on one
+ * device (two threads, two handles) race and must be serialised by the
+ * caller.  Returns when the run is complete, as g4s_run. */
+#define G4S_NOALLOC_SCRATCH_BYTES (1 << 20)
+int g4s_run_noalloc(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster_dev, int32_t *counts_dev,
+                    const g4s_run_opts *opts);
 
 /* Device time of each step of the last g4s_run / g4s_run_stream, in ms:
  * the interval between the starts of consecutive steps on the device's
