@@ -20,6 +20,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+CPU_CAP = 16  # threads of the CPU baseline: one GPU's share of the host
 
 # dominant kernel of each (scheme, mode): the one launch that carries a step
 KERNEL_OF = {("SKL", "atomic"): "k_skl_slice", ("AB", "atomic"): "k_slice_step", ("AP", "atomic"): "k_ap",
@@ -35,21 +36,29 @@ def step_bytes(n, nsyn, spikes):
 
 
 def profiled_traffic(kernel, alg, accum, n, nsyn, regime, slices, block):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summary of the same workload and plan, or None."""
-    return traffic_of(f"{kernel}|{alg}|{accum}|n={n}|nsyn={nsyn}|regime={regime}|slices={slices}|block={block}")
+    """The committed rocprofv3 FETCH_SIZE/WRITE_SIZE record of `kernel` on the
+    same workload and plan (profiles/traffic.json), or None."""
+    return traffic_record(f"{kernel}|{alg}|{accum}|n={n}|nsyn={nsyn}|regime={regime}|slices={slices}|block={block}")
 
 
-def traffic_of(key):
+def traffic_record(key):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
-            rec = json.load(f).get(key)
+            return json.load(f).get(key)
     except (OSError, ValueError):
         return None
-    if rec is None:
-        return None
-    return rec["bytes_per_step"] if "bytes_per_step" in rec else rec["bytes_per_launch"]
+
+
+def traffic_for_window(rec, alg_bytes_per_step, steps_per_launch):
+    """HBM bytes per launch of this window from a profiled record: the
+    profiled run's measured bytes per step over its algorithmic bytes per
+    step (which carry its own spikes per step), times this window's
+    algorithmic bytes per step and steps per launch."""
+    if not rec or "bytes_per_step" not in rec or "alg_bytes_per_step" not in rec:
+        return None, None
+    ratio = rec["bytes_per_step"] / rec["alg_bytes_per_step"]
+    return ratio * alg_bytes_per_step * steps_per_launch, ratio
 
 
 def kernel_bytes(alg, accum, n, nsyn, spikes):
@@ -68,7 +77,8 @@ def cpu_baseline(n, nsyn, seed, regime, steps):
     range of neurons and of targets, results bit-identical to one thread --
     and on one core."""
     from oracle import pyoracle
-    cores = max(1, min(16, os.cpu_count() or 1))
+    host = os.cpu_count() or 1
+    cores = max(1, min(CPU_CAP, host))
     net = pyoracle.Network(n, nsyn, seed, regime=regime)
     t0 = time.perf_counter()
     _, counts, _ = net.run(steps, raster=False, threads=cores)
@@ -79,7 +89,15 @@ def cpu_baseline(n, nsyn, seed, regime, steps):
     _, c1, _ = net1.run(one_steps, raster=False)
     dt1 = time.perf_counter() - t1
     ev = float(counts.sum()) * nsyn
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": ev / dt, "unit": "synaptic events/s", "cores": cores, "kind": "port",
+            "host_cores": host, "cpu_model": model,
+            "cap": f"at most {CPU_CAP} threads: the host share of one GPU on the benchmark machine",
             "sample": f"first {steps} steps of the same N={n} Nsyn={nsyn} network ({int(counts.sum())} spikes, "
                       f"{dt:.2f} s on {cores} threads of the C oracle)",
             "ms_per_sim_second": dt * 1000.0 / steps * 1000.0,
@@ -171,8 +189,9 @@ def bench_batch(g, args):
         "speedup_vs_plain": (sim_s / wall) / (p_sim / p_wall),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic_of(f"k_batch_step|batch{len(specs)}|{args.accum}|chunk={args.chunk}|slice_len=768"
-                                           f"|streams={args.streams or 3}|per_step")
+                     "traffic": (traffic_record(f"k_batch_step|batch{len(specs)}|{args.accum}|chunk={args.chunk}"
+                                                f"|slice_len=768|streams={args.streams or 3}|per_step") or {})
+                     .get("bytes_per_step")
                      if not args.slice_len and not args.tune and args.schedule == "step" and args.seed == 20221 else None,
                      "kernel": "k_batch_step",
                      # each step is one launch per instance group, the groups'
@@ -254,7 +273,7 @@ def bench_serve(g, args):
 def extras(g, args):
     """Batched and served summaries beside the single-network line, in the
     same accumulation mode: 256 networks in one batch (1000 steps), then
-    open-loop serving at 50 req/s for 20 s and an overload run at 400 req/s
+    open-loop serving at 50 req/s for 120 s and an overload run at 400 req/s
     for 10 s."""
     import copy
     t0 = time.perf_counter()
@@ -263,7 +282,7 @@ def extras(g, args):
     b = bench_batch(g, a)
     print(f"batched done in {time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
     s1 = copy.copy(a)
-    s1.serve, s1.rate = 20.0, 50.0
+    s1.serve, s1.rate = 120.0, 50.0
     sv = bench_serve(g, s1)
     s2 = copy.copy(a)
     s2.serve, s2.rate = 10.0, 400.0
@@ -320,6 +339,9 @@ def main():
     ap.add_argument("--tune", action="store_true", help="batched: tune the slice length per (N, Nsyn) bucket first")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched (256 networks) and served (Poisson arrivals) summaries")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="time only the headline window (no raster, kernel-event or zero-spike legs, no "
+                         "extras, no CPU baseline): the command profiled for the counter summaries")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times with a plain copy kernel "
                          "(reference reading for the FETCH_SIZE/WRITE_SIZE counters) and exit")
@@ -381,26 +403,57 @@ def main():
         print(f"autotune: {plan_info}", file=sys.stderr, flush=True)
     else:
         alg, block, slices = args.alg, args.block, args.slices
-    net.reset()
-    net.run(args.warmup, alg=alg, accum=args.accum, raster=False, block=block, slices=slices)
-    t0 = time.perf_counter()
-    _, counts = net.run(args.steps, alg=alg, accum=args.accum, raster=False, block=block, slices=slices)
-    wall = time.perf_counter() - t0
-    tm = net.timing()
+    kw = dict(alg=alg, accum=args.accum, block=block, slices=slices)
+
+    def window(raster=False, kernel_events=False):
+        """The timed window: from reset, args.warmup untimed steps, then
+        args.steps steps; returns (wall seconds, counts, timing)."""
+        net.reset()
+        net.run(args.warmup, raster=False, **kw)
+        t = time.perf_counter()
+        _, c = net.run(args.steps, raster=raster, kernel_events=kernel_events, **kw)
+        return time.perf_counter() - t, c, net.timing()
+
+    wall, counts, tm = window()
     spikes = int(counts.sum())
     events = spikes * args.nsyn
-
-    # Roofline of the dominant kernel: the same number of steps again, each
-    # launch of that kernel bracketed by HIP events on the library's stream.
-    _, kc = net.run(args.steps, alg=alg, accum=args.accum, raster=False, block=block, slices=slices,
-                    kernel_events=True)
-    tk = net.timing()
+    if args.headline_only:
+        print(json.dumps({"metric": "synaptic events/s, headline window only", "value": events / wall,
+                          "unit": "synaptic events/s", "steps": args.steps, "warmup": args.warmup,
+                          "ms_per_step": wall * 1e3 / args.steps, "device_ms_per_step": tm["device_ms"] / args.steps,
+                          "scheme": alg, "accum": args.accum, "block": block, "slices": slices,
+                          "spikes_per_step": spikes / args.steps,
+                          "kernel_bytes_per_step": sum(kernel_bytes(alg, args.accum, args.n, args.nsyn, int(x))
+                                                       for x in counts) / args.steps,
+                          "step_bytes_per_step": sum(step_bytes(args.n, args.nsyn, int(x))
+                                                     for x in counts) / args.steps,
+                          "launch": net.launch_info()}), flush=True)
+        net.close()
+        return
+    # the same window with the raster rows copied to host memory
+    wall_r, _, _ = window(raster=True)
+    # Roofline of the dominant kernel: the same window again, each launch of
+    # that kernel bracketed by events stamped by its own dispatch packet on
+    # the library's stream
+    _, kc, tk = window(kernel_events=True)
     kb = sum(kernel_bytes(alg, args.accum, args.n, args.nsyn, int(x)) for x in kc)
     k_launches = max(1, tk["kernel_launches"])
     per_launch_bytes = kb / k_launches
     avg_launch_s = tk["kernel_ms"] / 1e3 / k_launches
     achieved = per_launch_bytes / avg_launch_s / 1e9
     kname = KERNEL_OF.get((alg, args.accum), "step")
+    steps_per_launch = args.steps / k_launches
+    traffic, traffic_ratio = traffic_for_window(
+        profiled_traffic(kname, alg, args.accum, args.n, args.nsyn, args.regime, slices, block),
+        kb / args.steps, steps_per_launch)
+    # The step's floor: the same window with the thalamic input off (no
+    # spikes), i.e. update, bitmap scan and grid barrier without propagation
+    net.set_regime(args.regime, 0.0)
+    _, c0, t0m = window()
+    net.set_regime(args.regime, 1.0)
+    floor_us = t0m["device_ms"] * 1e3 / args.steps
+    step_us = tm["device_ms"] * 1e3 / args.steps
+    linfo = net.launch_info()
 
     out = {
         "metric": "synaptic events/s + ms per sim-second, N=100k; p50/p99 req latency (served)",
@@ -422,12 +475,19 @@ def main():
                    "scheme": alg, "accum": args.accum, "block": block, "slices": slices,
                    "autotune": plan_info, "spikes_per_step": spikes / args.steps,
                    "firing_rate_hz": spikes / args.steps / args.n * 1000.0, "seed": args.seed},
+        "ms_per_step_with_raster": wall_r * 1e3 / args.steps,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": profiled_traffic(kname, alg, args.accum, args.n, args.nsyn, args.regime, slices, block),
-                     "kernel": kname,
-                     "bytes_per_launch": per_launch_bytes, "avg_launch_us": avg_launch_s * 1e6,
-                     "launches_timed": k_launches},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_ratio": traffic_ratio,
+                     "kernel": kname, "bytes_per_launch": per_launch_bytes, "avg_launch_us": avg_launch_s * 1e6,
+                     "steps_per_launch": steps_per_launch, "kernel_us_per_step": avg_launch_s * 1e6 / steps_per_launch,
+                     "launches_timed": k_launches, "window": f"steps {args.warmup}..{args.warmup + args.steps} from reset",
+                     "launch": linfo},
+        "step_decomposition": {"device_us_per_step": step_us, "zero_spike_us_per_step": floor_us,
+                               "propagation_us_per_step": step_us - floor_us,
+                               "zero_spike_spikes": int(c0.sum()),
+                               "note": "zero_spike: the same plan and window with the thalamic input off (update, "
+                                       "bitmap scan, grid barrier); 40% of the HBM roofline would need "
+                                       f"{kb / args.steps / (0.4 * HBM_PEAK_GBS * 1e9) * 1e6:.2f} us per step"},
     }
     net.close()
     if not args.no_extras:

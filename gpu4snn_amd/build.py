@@ -137,7 +137,9 @@ def build(verbose: bool = False, arch: str = "gfx950") -> None:
     gen_obj = gen + ".o"
     _run([cc, f"--offload-arch={arch}", *COMMON, "-I", CSRC, "-x", "hip", "-c", gen, "-o", gen_obj], verbose)
     tmp = LIB + ".tmp"
-    _run([cc, f"--offload-arch={arch}", "-shared", "-fPIC", "-o", tmp, *objs, gen_obj, "-lpthread"], verbose)
+    rocm_lib = os.path.join(os.path.dirname(os.path.dirname(os.path.realpath(cc))), "lib")
+    _run([cc, f"--offload-arch={arch}", "-shared", "-fPIC", "-o", tmp, *objs, gen_obj, "-lpthread",
+          f"-L{rocm_lib}", "-lrocprofiler-sdk-roctx", f"-Wl,-rpath,{rocm_lib}"], verbose)
     os.replace(tmp, LIB)
     cli_srcs = [os.path.join(CSRC, s) for s in CLI_SOURCES if os.path.exists(os.path.join(CSRC, s))]
     if cli_srcs:

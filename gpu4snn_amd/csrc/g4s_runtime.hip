@@ -8,6 +8,8 @@
 #include "g4s_build.h"
 #include "g4s_internal.h"
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include <algorithm>
 #include <chrono>
 #include <cstdarg>
@@ -569,6 +571,13 @@ void record_launch(g4s_net *net, int alg, bool slice, int block, int grid, bool 
     }
 }
 
+// Capture range of a profiler run with --selected-regions, closed on every
+// return path.
+struct ProfileRange {
+    ProfileRange() { roctxProfilerResume(0); }
+    ~ProfileRange() { roctxProfilerPause(0); }
+};
+
 // Outputs of g4s_run_noalloc: device buffers of the caller.
 struct NoAllocOut {
     uint32_t *raster;  // steps x words, or NULL
@@ -716,6 +725,11 @@ int run_impl(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, 
         HIP_OK(launch_fill_u32((uint32_t *)d.err, 1, 0u, net->stream));
     }
 
+    // The step loop is the profiler's capture range (rocprofv3
+    // --selected-regions), as cudaProfilerStart/Stop bracket the reference's
+    // simulation (cu:1209, cu:1302): setup such as the connectivity build
+    // stays out of the trace.
+    ProfileRange prange;
     const auto wall0 = std::chrono::steady_clock::now();
     double dev_ms = 0.0, kern_ms = 0.0;
     int64_t spikes = 0;
