@@ -121,15 +121,15 @@ def batch_specs(count, seed):
 def bench_batch(g, args):
     specs = batch_specs(args.batch, args.seed)
     total = args.warmup + args.steps
-    tuned = {}
-    if args.tune and not args.slice_len:
+    tuned = None
+    if args.tune and not args.slice_len and args.schedule == "step":
+        # the batch autotuner on this very mix (same seeds), before the timed
+        # batch is built; its per-bucket slice lengths are then used by add()
         t_tune = time.perf_counter()
-        for (n, ns, sd, sc) in specs:
-            key = (n.bit_length() - 1, ns)
-            if key not in tuned:
-                tuned[key] = g.batch_tune(n, ns, sd, accum=args.accum, thal_scale=1.15, trial_steps=60)["slice_len"]
-        print(f"tuned {len(tuned)} buckets in {time.perf_counter() - t_tune:.1f} s: "
-              f"{sorted(tuned.items())}", file=sys.stderr, flush=True)
+        tuned = g.batch_tune_mix(specs, accum=args.accum, trial_steps=400, max_trials=args.tune_trials,
+                                 chunk=args.chunk, streams=args.streams, block=args.batch_block)
+        tuned["tune_s"] = time.perf_counter() - t_tune
+        print(f"mix tuner: {tuned}", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     b = g.Batch(args.accum, chunk=args.chunk, slice_len=args.slice_len, schedule=args.schedule,
                 block=args.batch_block, streams=args.streams)
@@ -179,8 +179,8 @@ def bench_batch(g, args):
         "config": {"workload": f"{len(specs)} networks, N log-uniform [1e3, 5e4], Nsyn in {{100,500,1000}}, "
                                f"thalamic x U(0.8,1.5), {steps_timed} timed steps each",
                    "accum": args.accum, "chunk": args.chunk,
-                   "slice_len": args.slice_len or ("tuned per bucket" if tuned else "min(768, N)"),
-                   "tuned": {f"log2N={k[0]},Nsyn={k[1]}": v for k, v in sorted(tuned.items())} or None,
+                   "slice_len": args.slice_len or ("tuned per bucket on the mix" if tuned else "min(768, N)"),
+                   "tuned": tuned,
                    "schedule": args.schedule, "block": args.batch_block or 256, "streams": args.streams or 3,
                    "neurons": sum(x[0] for x in specs),
                    "synapses": sum(x[0] * x[1] for x in specs), "seed": args.seed},
@@ -278,7 +278,7 @@ def extras(g, args):
     import copy
     t0 = time.perf_counter()
     a = copy.copy(args)
-    a.batch, a.steps, a.warmup, a.plain_sample, a.slice_len, a.tune = 256, 1000, 100, 8, 0, False
+    a.batch, a.steps, a.warmup, a.plain_sample, a.slice_len, a.tune = 256, 1000, 100, 8, 0, True
     b = bench_batch(g, a)
     print(f"batched done in {time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
     s1 = copy.copy(a)
@@ -336,7 +336,9 @@ def main():
     ap.add_argument("--streams", type=int, default=0,
                     help="batched: instance groups stepped on their own streams, 0 = the library default")
     ap.add_argument("--plain-sample", type=int, default=32)
-    ap.add_argument("--tune", action="store_true", help="batched: tune the slice length per (N, Nsyn) bucket first")
+    ap.add_argument("--tune", action="store_true",
+                    help="batched: tune the slice length per (N, Nsyn, drive) bucket on the mix first")
+    ap.add_argument("--tune-trials", type=int, default=40, help="batched: trials of the mix tuner")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched (256 networks) and served (Poisson arrivals) summaries")
     ap.add_argument("--headline-only", action="store_true",

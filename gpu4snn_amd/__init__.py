@@ -73,6 +73,12 @@ class BatchPlan(C.Structure):
                 ("rate_hz", C.c_float)]
 
 
+class MixPlan(C.Structure):
+    _fields_ = [("buckets", C.c_int32), ("changed", C.c_int32), ("trials", C.c_int32),
+                ("us_per_step_default", C.c_float), ("us_per_step_tuned", C.c_float),
+                ("us_per_step_first", C.c_float)]
+
+
 class InstInfo(C.Structure):
     _fields_ = [("n", C.c_int64), ("nsyn", C.c_int32), ("slice_len", C.c_int32), ("t", C.c_int64),
                 ("steps", C.c_int64), ("spikes", C.c_int64), ("bytes", C.c_int64), ("done", C.c_int32),
@@ -144,6 +150,8 @@ def lib():
             "g4s_batch_timing": ([_vp, C.POINTER(Timing)], C.c_int),
             "g4s_batch_destroy": ([_vp], C.c_int),
             "g4s_batch_tune": ([C.c_int, C.POINTER(Spec), C.c_int32, C.c_int, C.POINTER(BatchPlan)], C.c_int),
+            "g4s_batch_tune_mix": ([C.c_int, C.POINTER(Spec), C.c_int32, C.c_int32, C.POINTER(BatchOpts), C.c_int32,
+                                    C.POINTER(MixPlan)], C.c_int),
             "g4s_server_create": ([C.POINTER(ServerOpts), C.POINTER(_vp)], C.c_int),
             "g4s_server_submit": ([_vp, C.POINTER(Spec), C.c_int64, C.c_int, C.POINTER(C.c_int64)], C.c_int),
             "g4s_server_info": ([_vp, C.c_int64, C.POINTER(RequestInfo)], C.c_int),
@@ -359,6 +367,19 @@ def footprint(n, nsyn, chunk=0, slice_len=0, seed=0, regime=1, thal_scale=1.0):
     if v < 0:
         raise G4SError(-1, lib().g4s_last_error().decode())
     return v
+
+
+def batch_tune_mix(specs, accum="atomic", trial_steps=100, max_trials=40, chunk=0, streams=0, block=0):
+    """g4s_batch_tune_mix on a mix of networks given as (N, Nsyn, seed,
+    thal_scale) tuples (balanced regime); returns the plan summary.  The
+    tuned slice lengths are recorded per bucket and used by later Batch.add
+    calls of batches with slice_len 0."""
+    arr = (Spec * len(specs))(*[make_spec(n, ns, sd, 1, sc) for (n, ns, sd, sc) in specs])
+    opts = BatchOpts(chunk, 0, 0, 0, 0, block, streams)
+    p = MixPlan()
+    _check(lib().g4s_batch_tune_mix(ACCUM[accum], arr, len(specs), int(trial_steps), C.byref(opts), int(max_trials),
+                                    C.byref(p)))
+    return {k: getattr(p, k) for k, _ in MixPlan._fields_}
 
 
 def batch_tune(n, nsyn, seed=1, accum="det", thal_scale=1.0, trial_steps=100, force=False):

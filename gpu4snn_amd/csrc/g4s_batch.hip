@@ -199,15 +199,19 @@ int check_slot(const g4s_batch *b, int32_t slot)
     return G4S_OK;
 }
 
-// Tuned slice lengths per bucket (floor(log2 N), Nsyn, accum), filled by
-// g4s_batch_tune for the life of the process.
+// Tuned slice lengths per bucket (floor(log2 N), Nsyn, drive class, accum),
+// filled by g4s_batch_tune / g4s_batch_tune_mix for the life of the
+// process.  The drive class -- regime and thalamic scale, which set the
+// firing rate -- stands for the rate bucket: it is known when an instance is
+// added, the rate only after it has run.
 struct TunedLen {
     int slice_len;
     float us_per_inst_step;
     float rate_hz;
 };
+using BucketKey = std::tuple<int, int, int, int>;
 std::mutex g_tune_mu;
-std::map<std::tuple<int, int, int>, TunedLen> g_tuned;
+std::map<BucketKey, TunedLen> g_tuned;
 
 int ilog2i(int64_t x)
 {
@@ -219,15 +223,29 @@ int ilog2i(int64_t x)
     return r;
 }
 
-int slice_len_for(int64_t n, int32_t nsyn, int32_t requested, int accum)
+// Drive class: regime and thalamic scale in quarters (scale 0.8-1.5 of the
+// balanced regime falls in classes 3, 4 and 5).
+int drive_class(const g4s_spec *s)
 {
+    const float q = std::min(std::max(s->thal_scale, 0.0f), 64.0f) * 4.0f;
+    return s->regime * 1024 + (int)q;
+}
+
+BucketKey bucket_of(const g4s_spec *s, int accum)
+{
+    return BucketKey(ilog2i(s->n), (int)s->nsyn, drive_class(s), accum);
+}
+
+int slice_len_for(const g4s_spec *s, int32_t requested, int accum)
+{
+    const int64_t n = s->n;
     if (requested > 0) return requested;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
-        auto it = g_tuned.find(std::make_tuple(ilog2i(n), (int)nsyn, accum));
+        auto it = g_tuned.find(bucket_of(s, accum));
         if (it != g_tuned.end()) return (int)std::min<int64_t>(it->second.slice_len, (n + 63) / 64 * 64);
     }
-    return default_slice_len(n, nsyn);
+    return default_slice_len(n, s->nsyn);
 }
 
 }  // namespace
@@ -238,7 +256,7 @@ int64_t g4s_batch_footprint(const g4s_spec *spec, int32_t chunk, int32_t slice_l
 {
     if (validate_spec(spec)) return -1;
     const int64_t n = spec->n;
-    const int L = slice_len_for(n, spec->nsyn, slice_len, -1);
+    const int L = slice_len_for(spec, slice_len, -1);
     const int64_t P = (n + L - 1) / L;
     const int64_t words = (n + 31) / 32;
     const int cs = chunk > 0 ? chunk : kDefaultChunk;
@@ -286,7 +304,11 @@ int g4s_batch_create(int accum, const g4s_batch_opts *opts, g4s_batch **out)
     return G4S_OK;
 }
 
-int g4s_batch_add(g4s_batch *b, const g4s_spec *spec, int64_t steps, int keep_raster, int32_t *slot)
+}  // extern "C"
+
+namespace {
+// g4s_batch_add with the slice length given (0: the batch's rule).
+int batch_add(g4s_batch *b, const g4s_spec *spec, int64_t steps, int keep_raster, int32_t slice_len, int32_t *slot)
 {
     if (!b || !slot) return fail(G4S_ERR_ARG, "NULL argument");
     int rc = validate_spec(spec);
@@ -294,7 +316,7 @@ int g4s_batch_add(g4s_batch *b, const g4s_spec *spec, int64_t steps, int keep_ra
     if (steps < 0 || steps > 0x7FFFFFFFLL) return fail(G4S_ERR_ARG, "steps out of range");
     if (spec->nsyn > 65535) return fail(G4S_ERR_UNSUPPORTED, "batched instances need Nsyn < 65536");
     const int64_t n = spec->n;
-    const int L = slice_len_for(n, spec->nsyn, b->opts.slice_len, b->accum);
+    const int L = slice_len > 0 ? slice_len : slice_len_for(spec, b->opts.slice_len, b->accum);
     const bool det = b->accum == G4S_ACCUM_DET;
     if (batch_lds(L, det, b->block) > 64 * 1024)
         return fail(G4S_ERR_UNSUPPORTED, "slice length %d needs %zu bytes of LDS", L, batch_lds(L, det, b->block));
@@ -367,6 +389,14 @@ int g4s_batch_add(g4s_batch *b, const g4s_spec *spec, int64_t steps, int keep_ra
     b->bytes_in_use += fp;
     *slot = s;
     return G4S_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int g4s_batch_add(g4s_batch *b, const g4s_spec *spec, int64_t steps, int keep_raster, int32_t *slot)
+{
+    return batch_add(b, spec, steps, keep_raster, 0, slot);
 }
 
 int g4s_batch_step(g4s_batch *b, int32_t *active)
@@ -709,7 +739,7 @@ int g4s_batch_tune(int accum, const g4s_spec *rep, int32_t trial_steps, int forc
     if (!out) return fail(G4S_ERR_ARG, "out is NULL");
     if (accum != G4S_ACCUM_ATOMIC && accum != G4S_ACCUM_DET) return fail(G4S_ERR_ARG, "unknown accum %d", accum);
     if (trial_steps < 10) trial_steps = 100;
-    const auto key = std::make_tuple(ilog2i(rep->n), (int)rep->nsyn, accum);
+    const BucketKey key = bucket_of(rep, accum);
     if (!force) {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tuned.find(key);
@@ -782,6 +812,160 @@ int g4s_batch_tune(int accum, const g4s_spec *rep, int32_t trial_steps, int forc
     out->copies = copies;
     return G4S_OK;
 }
+
+}  // extern "C"
+
+namespace {
+// Device time per step of the whole mix `specs`, each instance at the slice
+// length of its bucket in `len`: a batch built from the same seeds, `warm`
+// steps, then `steps` timed steps.
+int time_mix(int accum, const g4s_batch_opts &o, const std::vector<g4s_spec> &specs,
+             const std::vector<BucketKey> &keys, const std::map<BucketKey, int> &len, int warm, int steps, double *us,
+             std::vector<int64_t> *spikes)
+{
+    g4s_batch_opts bo = o;
+    bo.chunk = std::max(warm, steps);
+    g4s_batch *b = nullptr;
+    int rc = g4s_batch_create(accum, &bo, &b);
+    if (rc) return rc;
+    std::vector<int32_t> slots(specs.size());
+    for (size_t k = 0; k < specs.size() && rc == G4S_OK; ++k)
+        rc = batch_add(b, &specs[k], warm + steps, 0, len.at(keys[k]), &slots[k]);
+    int32_t act = 0;
+    if (rc == G4S_OK) {
+        b->chunk = warm;
+        rc = g4s_batch_step(b, &act);
+    }
+    if (rc == G4S_OK) {
+        b->chunk = steps;
+        rc = g4s_batch_step(b, &act);
+    }
+    if (rc == G4S_OK) {
+        *us = b->timing.device_ms * 1000.0 / steps;
+        if (spikes) {
+            spikes->assign(specs.size(), 0);
+            for (size_t k = 0; k < specs.size(); ++k) (*spikes)[k] = b->inst[(size_t)slots[k]].spikes;
+        }
+    }
+    g4s_batch_destroy(b);
+    return rc;
+}
+}  // namespace
+
+extern "C" int g4s_batch_tune_mix(int accum, const g4s_spec *specs, int32_t count, int32_t trial_steps,
+                                  const g4s_batch_opts *opts, int32_t max_trials, g4s_mix_plan *out)
+{
+    if (!specs || count <= 0 || !out) return fail(G4S_ERR_ARG, "bad arguments");
+    if (accum != G4S_ACCUM_ATOMIC && accum != G4S_ACCUM_DET) return fail(G4S_ERR_ARG, "unknown accum %d", accum);
+    g4s_batch_opts o{};
+    if (opts) o = *opts;
+    if (o.schedule != 0 || o.slice_len != 0) return fail(G4S_ERR_ARG, "tuning needs schedule 0 and slice_len 0");
+    if (trial_steps < 10) trial_steps = 100;
+    if (max_trials <= 0) max_trials = 40;
+    std::vector<g4s_spec> sp(specs, specs + count);
+    std::vector<BucketKey> keys((size_t)count);
+    std::map<BucketKey, int> len;        // current plan
+    std::map<BucketKey, double> weight;  // synapses of the bucket: its share of the work
+    for (int32_t k = 0; k < count; ++k) {
+        int rc = validate_spec(&sp[(size_t)k]);
+        if (rc) return rc;
+        keys[(size_t)k] = bucket_of(&sp[(size_t)k], accum);
+        len[keys[(size_t)k]] = default_slice_len(sp[(size_t)k].n, sp[(size_t)k].nsyn);
+        weight[keys[(size_t)k]] += (double)sp[(size_t)k].n * (double)sp[(size_t)k].nsyn;
+    }
+    // the largest bucket's lengths cover all: a bucket's candidates may not
+    // exceed its largest N (rounded up to 64)
+    std::map<BucketKey, int64_t> nmax;
+    for (int32_t k = 0; k < count; ++k)
+        nmax[keys[(size_t)k]] = std::max<int64_t>(nmax[keys[(size_t)k]], sp[(size_t)k].n);
+    const int warm = std::max(20, trial_steps);
+    int trials = 0;
+    double base = 0.0;
+    std::vector<int64_t> spk;
+    int rc = time_mix(accum, o, sp, keys, len, warm, trial_steps, &base, &spk);
+    if (rc) return rc;
+    ++trials;
+    const double start = base;
+    // buckets by weight, heaviest first
+    std::vector<std::pair<double, BucketKey>> order;
+    for (auto &kv : weight) order.emplace_back(-kv.second, kv.first);
+    std::sort(order.begin(), order.end());
+    constexpr int kLens[] = {256, 384, 512, 640, 768, 1024, 1280, 1536, 2048};
+    constexpr int kNL = (int)(sizeof(kLens) / sizeof(kLens[0]));
+    const bool det = accum == G4S_ACCUM_DET;
+    int changed = 0;
+    for (auto &ob : order) {
+        const BucketKey key = ob.second;
+        const int64_t cap = (nmax[key] + 63) / 64 * 64;
+        int idx = 0;
+        while (idx + 1 < kNL && kLens[idx] < len[key]) ++idx;
+        // step down, then up; follow an improving direction while it improves
+        for (int dir : {-1, 1}) {
+            bool moved = false;
+            for (int j = idx + dir; j >= 0 && j < kNL && trials < max_trials; j += dir) {
+                const int L = kLens[j];
+                if (L > cap && kLens[j - dir] >= cap) break;  // no longer changes the slicing
+                if (batch_lds(L, det) > 64 * 1024) break;
+                const int old = len[key];
+                len[key] = L;
+                double us = 0.0;
+                rc = time_mix(accum, o, sp, keys, len, warm, trial_steps, &us, nullptr);
+                ++trials;
+                if (rc == G4S_ERR_UNSUPPORTED || rc == G4S_ERR_MEM) rc = G4S_OK, us = INFINITY;
+                if (rc) return rc;
+                if (us < base * 0.985) {  // a change must beat timing noise
+                    base = us;
+                    idx = j;
+                    moved = true;
+                    ++changed;
+                } else {
+                    len[key] = old;
+                    break;
+                }
+            }
+            if (moved) break;
+            if (trials >= max_trials) break;
+        }
+        if (trials >= max_trials) break;
+    }
+    // confirm against the untuned plan once more; keep the tuned plan only
+    // if it still wins
+    std::map<BucketKey, int> dflt;
+    for (int32_t k = 0; k < count; ++k)
+        dflt[keys[(size_t)k]] = default_slice_len(sp[(size_t)k].n, sp[(size_t)k].nsyn);
+    double us_t = 0.0, us_d = 0.0;
+    for (int rep = 0; rep < 2; ++rep) {  // alternated, summed: drift hits both alike
+        double a = 0.0, c = 0.0;
+        if ((rc = time_mix(accum, o, sp, keys, len, warm, trial_steps, &a, nullptr))) return rc;
+        if ((rc = time_mix(accum, o, sp, keys, dflt, warm, trial_steps, &c, nullptr))) return rc;
+        us_t += a / 2;
+        us_d += c / 2;
+        trials += 2;
+    }
+    const bool keep = us_t < us_d;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        for (auto &kv : (keep ? len : dflt)) {
+            double spikes = 0.0, neurons = 0.0;
+            for (int32_t k = 0; k < count; ++k)
+                if (keys[(size_t)k] == kv.first) {
+                    spikes += (double)spk[(size_t)k];
+                    neurons += (double)sp[(size_t)k].n;
+                }
+            g_tuned[kv.first] = TunedLen{kv.second, (float)(keep ? us_t : us_d),
+                                         (float)(spikes / std::max(1.0, neurons) / (warm + trial_steps) * 1000.0)};
+        }
+    }
+    out->buckets = (int32_t)len.size();
+    out->changed = keep ? changed : 0;
+    out->trials = trials;
+    out->us_per_step_default = (float)us_d;
+    out->us_per_step_tuned = (float)(keep ? us_t : us_d);
+    out->us_per_step_first = (float)start;
+    return G4S_OK;
+}
+
+extern "C" {
 
 int g4s_batch_destroy(g4s_batch *b)
 {

@@ -307,13 +307,16 @@ int g4s_batch_remove(g4s_batch *b, int32_t slot);
 int g4s_batch_timing(const g4s_batch *b, g4s_timing *out);
 int g4s_batch_destroy(g4s_batch *b);
 
-/* Batch autotuner (no reference counterpart).  Times the slice lengths 128
- * .. 2048 on a batch of copies of `rep` (different seeds; enough copies to
- * fill the device, so throughput decides) for trial_steps steps after a
- * warm-up, and records the fastest for the bucket (floor(log2 N), Nsyn,
- * accum) for the life of the process: g4s_batch_add then uses it for every
- * instance in that bucket when the batch's slice_len is 0.  A recorded
- * bucket is returned without timing unless `force`. */
+/* Batch autotuner on one network shape (no reference counterpart).  Times
+ * the slice lengths 128 .. 2048 on a batch of copies of `rep` (different
+ * seeds; enough copies to fill the device, so throughput decides) for
+ * trial_steps steps after a warm-up, and records the fastest for the bucket
+ * (floor(log2 N), Nsyn, drive class, accum; see g4s_batch_tune_mix) for the
+ * life of the process: g4s_batch_add then uses it for every instance in that
+ * bucket when the batch's slice_len is 0.  A recorded bucket is returned
+ * without timing unless `force`.  On a mix of shapes g4s_batch_tune_mix
+ * judges better: a homogeneous trial does not see which instances set the
+ * mix's step time. */
 typedef struct g4s_batch_plan {
     int32_t slice_len;         /* chosen neurons per slice                  */
     int32_t copies;            /* instances timed (0: cached plan)           */
@@ -321,6 +324,32 @@ typedef struct g4s_batch_plan {
     float   rate_hz;           /* firing rate seen while timing              */
 } g4s_batch_plan;
 int g4s_batch_tune(int accum, const g4s_spec *rep, int32_t trial_steps, int force, g4s_batch_plan *out);
+
+/* Batch autotuner on the mix itself (no reference counterpart).  Times the
+ * whole batch of `specs` -- built from the same seeds for every trial, with
+ * the batch options `opts` (schedule 0, slice_len 0) -- for trial_steps
+ * steps after as many warm-up steps, and by coordinate descent over the
+ * buckets (floor(log2 N), Nsyn, drive class = regime and thalamic scale in
+ * quarters, which set the firing rate), heaviest bucket (most synapses)
+ * first, moves one bucket's slice length at a time through 256 .. 2048 while
+ * the mix's device time per step drops by more than 1.5%, within max_trials
+ * trials (<= 0: 40).  The plan is then timed against the untuned one
+ * (every instance at min(768, N)), twice each, alternated, and kept only if
+ * faster: it is recorded
+ * per bucket for the life of the process, and g4s_batch_add with slice_len 0
+ * uses it.  The single-network autotuner (g4s_autotune) also picks the
+ * scheme; a batch runs every instance with the slice scheme, the only one
+ * that serves many networks in one launch. */
+typedef struct g4s_mix_plan {
+    int32_t buckets;             /* buckets in the mix                      */
+    int32_t changed;             /* buckets whose length the tuner moved    */
+    int32_t trials;              /* timed trials                            */
+    float   us_per_step_default; /* device us per step, every instance at min(768, N) */
+    float   us_per_step_tuned;   /* device us per step with the kept plan   */
+    float   us_per_step_first;   /* first trial (the starting plan)          */
+} g4s_mix_plan;
+int g4s_batch_tune_mix(int accum, const g4s_spec *specs, int32_t count, int32_t trial_steps,
+                       const g4s_batch_opts *opts, int32_t max_trials, g4s_mix_plan *out);
 
 /* ------------------------------------------------------------------------
  * Serving (no reference counterpart).  A server owns one batch and a worker

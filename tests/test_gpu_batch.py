@@ -247,3 +247,23 @@ def test_batched_workload_at_its_size(g4s, oracle):
                     assert np.array_equal(u.view(np.uint32), onet.u.view(np.uint32)), (k, n, ns)
                     assert np.array_equal(nos, onet.nospks), (k, n, ns)
     assert abs(totals["atomic"] - totals["det"]) <= 0.01 * totals["det"], totals
+
+
+def test_batch_tune_mix_plan_is_used_and_exact(g4s, oracle):
+    """g4s_batch_tune_mix times the mix itself, keeps its plan only when it
+    beats every instance at min(768, N), records it per bucket, and later
+    batches use it; det results do not depend on the slicing."""
+    rng = np.random.default_rng(9)
+    specs = []
+    for i in range(24):
+        n = int(np.exp(rng.uniform(np.log(1500), np.log(30000))))
+        specs.append((n, int(rng.choice([100, 500])), 500 + i, float(rng.uniform(0.8, 1.5))))
+    plan = g4s.batch_tune_mix(specs, accum="det", trial_steps=40, max_trials=8)
+    assert plan["trials"] >= 3 and plan["buckets"] >= 1
+    assert plan["us_per_step_tuned"] <= plan["us_per_step_default"] + 1e-6
+    n, ns, sd, sc = specs[0]
+    with g4s.Batch("det", chunk=50) as b:
+        s = b.add(n, ns, sd, 90, thal_scale=sc)
+        b.run()
+        assert b.info(s)["slice_len"] % 64 == 0
+        check_instance(g4s, oracle, b, s, n, ns, sd, sc, 90)
