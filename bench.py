@@ -411,7 +411,7 @@ def main():
         """The timed window: from reset, args.warmup untimed steps, then
         args.steps steps; returns (wall seconds, counts, timing)."""
         net.reset()
-        net.run(args.warmup, raster=False, **kw)
+        net.run(args.warmup, raster=raster, **kw)  # the raster leg's copies warm up too
         t = time.perf_counter()
         _, c = net.run(args.steps, raster=raster, kernel_events=kernel_events, **kw)
         return time.perf_counter() - t, c, net.timing()

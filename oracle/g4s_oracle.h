@@ -83,8 +83,11 @@ int orc_build_synapses(uint64_t seed, int64_t n, int64_t n_exc, int32_t nsyn,
  * receives 1 for each spike; counts (steps, may be NULL) the spikes per
  * step; lastI (may be NULL) the total input of the last step.
  * Pinned: hand-evaluated single steps, RS/FS textbook firing, brute-force
- * dense propagation on tiny N, and the reference notebook's spike totals at
- * N=2500, Nsyn=1000 for all three regimes. */
+ * dense propagation on tiny N, and the firing rates of an independent numpy
+ * model of the same network at N=2500, Nsyn=1000 in all three regimes
+ * (tests/golden/model_rates.json).  The reference notebook's spike totals
+ * (nb:297-614) are not a pin: they come from another revision of the
+ * reference source and fire about 5x faster than the current one. */
 typedef struct {
     int64_t n, n_exc;
     int32_t nsyn;
