@@ -26,6 +26,9 @@ constexpr int kCtlStride = 4;                 // per-step control words
 // Control row of a step: [0] spikes of the step (k_ap keeps finished
 // workgroups in word 1), [2..3] the step's start on the device wall clock.
 enum { kCtlCount = 0, kCtlStamp = 2 };
+// A run's control block: kCtlHead words (word 0 the error flag of bounded
+// waits) followed by the control rows, so that one copy brings both back.
+constexpr int kCtlHead = 4;
 constexpr uint32_t kSpinLimit = 1u << 24;     // bounded device waits (~10 s)
 // Grid-barrier arrival counters: one per group of workgroups (blockIdx % 16,
 // two groups per XCD), each in its own 256-byte line.  Measured at N=100k,
@@ -93,6 +96,8 @@ struct Launch {
 // Kernel launchers (g4s_kernels.hip).  Each returns the HIP status.
 hipError_t launch_init(const DevNet &d, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, size_t n, uint32_t val, hipStream_t s);
+// Two ranges filled by one launch.
+hipError_t launch_fill2_u32(uint32_t *p, size_t n, uint32_t *q, size_t m, uint32_t val, hipStream_t s);
 // counts[r] = spike count of control row r, r < steps.
 hipError_t launch_copy_counts(const int32_t *ctl, int32_t *counts, int steps, hipStream_t s);
 // Stamp the end of a chunk of `rel` steps into control row `rel`.
@@ -103,6 +108,7 @@ hipError_t launch_push_rows(const DevNet &d, int t, int rel, const Launch &L);
 // the pending input buffer when a global-atomic scheme follows one that
 // keeps none).
 hipError_t launch_push_bitmap(const DevNet &d, int t, hipStream_t s);
+// Also stamps the chunk's end into control row `steps`.
 hipError_t launch_reduce_counts(const DevNet &d, int steps, hipStream_t s);
 hipError_t launch_push_flat(const DevNet &d, int t, int rel, const Launch &L);
 hipError_t launch_ap(const DevNet &d, int t, int rel, int last, int prev_count, const Launch &L);

@@ -276,6 +276,17 @@ __global__ void __launch_bounds__(256) k_fill_u32(uint32_t *p, size_t n, uint32_
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = val;
 }
 
+__global__ void __launch_bounds__(256) k_fill2_u32(uint32_t *p, size_t n, uint32_t *q, size_t m, uint32_t val)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n + m; i += stride) {
+        if (i < n)
+            p[i] = val;
+        else
+            q[i - n] = val;
+    }
+}
+
 // counts[r] = spikes of chunk step r, r < steps (from the control rows).
 __global__ void __launch_bounds__(256) k_copy_counts(const int32_t *ctl, int32_t *counts, int steps)
 {
@@ -1013,9 +1024,12 @@ __global__ void __launch_bounds__(256) k_slice_offsets(DevNet d, uint16_t *soff)
 }
 
 // ctl[rel].count = sum over slices of bcnt[rel][p]; one workgroup per step.
+// It runs right after the chunk's step kernels: its start is the chunk's end
+// stamp (control row `steps`).
 __global__ void __launch_bounds__(256) k_reduce_counts(DevNet d, int steps)
 {
     __shared__ int part[256];
+    if (blockIdx.x == 0 && threadIdx.x == 0) stamp_step(d.ctl, steps);
     for (int rel = blockIdx.x; rel < steps; rel += gridDim.x) {
         int s = 0;
         const int32_t *row = d.bcnt + (size_t)rel * (size_t)d.P;
@@ -1594,6 +1608,13 @@ hipError_t launch_copy_counts(const int32_t *ctl, int32_t *counts, int steps, hi
 {
     if (steps <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_copy_counts, dim3(grid_for(steps, 256)), dim3(256), 0, s, ctl, counts, steps);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill2_u32(uint32_t *p, size_t n, uint32_t *q, size_t m, uint32_t val, hipStream_t s)
+{
+    if (n + m == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill2_u32, dim3(grid_for((int64_t)(n + m), 256)), dim3(256), 0, s, p, n, q, m, val);
     return hipGetLastError();
 }
 

@@ -15,7 +15,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -58,8 +60,8 @@ int scratch_of_current_device(unsigned char **out)
 }
 
 // Layout of the scratch array for a run of chunks of `cs` steps over P
-// slices: barrier counters, error flag, step info, control rows (cs + 1),
-// per-slice counts (cs x P).
+// slices: barrier counters, the control block (error flag, then cs + 1
+// control rows), step info, per-slice counts (cs x P).
 struct ScratchLayout {
     size_t bar, err, tinfo, ctl, bcnt, end;
 };
@@ -69,9 +71,9 @@ ScratchLayout scratch_layout(int cs, int P)
     auto up = [](size_t x) { return (x + 255) / 256 * 256; };
     l.bar = 0;
     l.err = up(l.bar + (size_t)kBarWords * 4);
-    l.tinfo = up(l.err + 4);
-    l.ctl = up(l.tinfo + 8);
-    l.bcnt = up(l.ctl + (size_t)(cs + 1) * kCtlStride * 4);
+    l.ctl = l.err + (size_t)kCtlHead * 4;  // the control block: error flag, then rows
+    l.tinfo = up(l.ctl + (size_t)(cs + 1) * kCtlStride * 4);
+    l.bcnt = up(l.tinfo + 8);
     l.end = l.bcnt + (size_t)cs * (size_t)std::max(P, 0) * 4;
     return l;
 }
@@ -150,8 +152,7 @@ struct g4s_net {
     int64_t t = 0;
     int chunk_cap = 0;             // raster rows / ctl entries allocated
     uint32_t *h_raster[2] = {nullptr, nullptr};
-    int32_t *h_ctl[2] = {nullptr, nullptr};
-    int32_t *h_err = nullptr;      // [2]
+    int32_t *h_ctl[2] = {nullptr, nullptr};  // control block copies: error flag, then the rows
     bool slices_ready = false;
     // Per-step AB launches captured once per chunk length and device layout.
     hipGraph_t graph = nullptr;
@@ -168,7 +169,13 @@ struct g4s_net {
     int persist_cache[1][17] = {};  // resident workgroups per CU per (kernel, block/64)
     int cus = 0;
     g4s_timing timing{};
-    g4s_launch_info linfo{};
+    // what the last run launched; g4s_get_launch_info builds its report
+    struct LastLaunch {
+        int alg = -1, block = 0, grid = 0;
+        bool slice = false, det = false;
+    } last_launch;
+    // resident workgroups per CU of SKL's instantiation per (L, block, det)
+    std::map<std::tuple<int, int, bool>, int> skl_res;
     // Per-step device time of the last run (ms), from the steps' start
     // stamps on the device wall clock (clock_khz).
     std::vector<float> step_ms;
@@ -237,17 +244,22 @@ int ensure_chunk(g4s_net *net, int chunk)
     net->chunk_cap = 0;
     net->gen += 1;
     if (net->d.raster) { (void)hipFree(net->d.raster); net->allocs.erase(std::find(net->allocs.begin(), net->allocs.end(), (void *)net->d.raster)); }
-    if (net->d.ctl) { (void)hipFree(net->d.ctl); net->allocs.erase(std::find(net->allocs.begin(), net->allocs.end(), (void *)net->d.ctl)); }
+    // the control block: error flag (d.err) and control rows (d.ctl) in one allocation
+    if (net->d.err) { (void)hipFree(net->d.err); net->allocs.erase(std::find(net->allocs.begin(), net->allocs.end(), (void *)net->d.err)); }
     for (int k = 0; k < 2; ++k) {
         if (net->h_raster[k]) (void)hipHostFree(net->h_raster[k]);
         if (net->h_ctl[k]) (void)hipHostFree(net->h_ctl[k]);
         net->h_raster[k] = nullptr;
         net->h_ctl[k] = nullptr;
     }
-    net->d.raster = nullptr; net->d.ctl = nullptr;
+    net->d.raster = nullptr; net->d.ctl = nullptr; net->d.err = nullptr;
     int rc;
     if ((rc = net->alloc(&net->d.raster, (size_t)chunk * net->d.words))) return rc;
-    if ((rc = net->alloc(&net->d.ctl, (size_t)(chunk + 1) * kCtlStride))) return rc;
+    int32_t *blk = nullptr;
+    if ((rc = net->alloc(&blk, (size_t)kCtlHead + (size_t)(chunk + 1) * kCtlStride))) return rc;
+    net->d.err = blk;
+    net->d.ctl = blk + kCtlHead;
+    HIP_OK(launch_fill_u32((uint32_t *)blk, kCtlHead, 0u, net->stream));
     if (net->d.bcnt) {
         (void)hipFree(net->d.bcnt);
         net->allocs.erase(std::find(net->allocs.begin(), net->allocs.end(), (void *)net->d.bcnt));
@@ -256,7 +268,7 @@ int ensure_chunk(g4s_net *net, int chunk)
     if (net->d.P > 0 && (rc = net->alloc(&net->d.bcnt, (size_t)chunk * net->d.P))) return rc;
     for (int k = 0; k < 2; ++k) {
         HIP_OK(hipHostMalloc((void **)&net->h_raster[k], (size_t)chunk * net->d.words * sizeof(uint32_t)));
-        HIP_OK(hipHostMalloc((void **)&net->h_ctl[k], (size_t)(chunk + 1) * kCtlStride * sizeof(int32_t)));
+        HIP_OK(hipHostMalloc((void **)&net->h_ctl[k], ((size_t)kCtlHead + (size_t)(chunk + 1) * kCtlStride) * sizeof(int32_t)));
     }
     net->chunk_cap = chunk;
     net->gen += 1;
@@ -469,12 +481,10 @@ int g4s_create(const g4s_spec *spec, g4s_net **out)
     if ((rc = net->alloc(&d.q0, n))) return bail(rc);
     if ((rc = net->alloc(&d.q1, n))) return bail(rc);
     if ((rc = net->alloc(&d.bar, kBarWords))) return bail(rc);
-    if ((rc = net->alloc(&d.err, 1))) return bail(rc);
     int32_t *tinfo = nullptr;
     if ((rc = net->alloc(&tinfo, 2))) return bail(rc);
     d.tinfo = tinfo;
-    if (hipHostMalloc((void **)&net->h_tinfo, 4 * sizeof(int32_t)) != hipSuccess ||
-        hipHostMalloc((void **)&net->h_err, 2 * sizeof(int32_t)) != hipSuccess)
+    if (hipHostMalloc((void **)&net->h_tinfo, 4 * sizeof(int32_t)) != hipSuccess)
         return bail(fail(G4S_ERR_MEM, "hipHostMalloc failed"));
     uint2 *syn = nullptr;
     const size_t ns = n * (size_t)spec->nsyn;
@@ -490,6 +500,7 @@ int g4s_create(const g4s_spec *spec, g4s_net **out)
 int g4s_reset(g4s_net *net)
 {
     if (!net) return fail(G4S_ERR_ARG, "net is NULL");
+    if (!net->d.err) return fail(G4S_ERR_STATE, "handle has no control block (a reallocation failed)");
     HIP_OK(launch_init(net->d, net->stream));
     HIP_OK(launch_fill_u32((uint32_t *)net->d.err, 1, 0u, net->stream));
     HIP_OK(hipStreamSynchronize(net->stream));
@@ -539,14 +550,39 @@ hipError_t launch_step(g4s_net *net, int alg, int block, int t, int rel, int las
 
 namespace {
 
-// Launch report of the dominant kernel of a run (g4s_get_launch_info).
+// SKL's resident workgroups per CU at slice length L (cached per handle:
+// the occupancy query is host work on every run otherwise).
+int skl_resident(g4s_net *net, int L, int block, bool det)
+{
+    const auto key = std::make_tuple(L, block, det);
+    auto it = net->skl_res.find(key);
+    if (it != net->skl_res.end()) return it->second;
+    const int r = skl_slice_blocks_per_cu(L, block, det);
+    if (r > 0) net->skl_res[key] = r;
+    return r;
+}
+
+// What a run launched, for g4s_get_launch_info.
 void record_launch(g4s_net *net, int alg, bool slice, int block, int grid, bool det)
 {
-    g4s_launch_info &li = net->linfo;
-    li = g4s_launch_info{};
+    net->last_launch.alg = alg;
+    net->last_launch.slice = slice;
+    net->last_launch.block = block;
+    net->last_launch.grid = grid;
+    net->last_launch.det = det;
+}
+
+// Launch report of the dominant kernel of the last run.
+g4s_launch_info launch_report(g4s_net *net)
+{
+    g4s_launch_info li{};
+    const int alg = net->last_launch.alg;
+    if (alg < 0) return li;
+    const bool slice = net->last_launch.slice, det = net->last_launch.det;
+    const int block = net->last_launch.block;
     if (net->cus == 0) net->cus = device_cus();
     li.cus = net->cus;
-    li.grid = grid;
+    li.grid = net->last_launch.grid;
     li.block = block;
     char targs[32] = "";
     const char *name;
@@ -554,7 +590,7 @@ void record_launch(g4s_net *net, int alg, bool slice, int block, int grid, bool 
         name = alg == G4S_ALG_SKL ? "k_skl_slice" : "k_slice_step";
         slice_kernel_targs(net->d.L, block, det, targs, sizeof(targs));
         li.lds_bytes = (int32_t)slice_lds(net->d.L, block, det);
-        li.resident_per_cu = alg == G4S_ALG_SKL ? skl_slice_blocks_per_cu(net->d.L, block, det)
+        li.resident_per_cu = alg == G4S_ALG_SKL ? skl_resident(net, net->d.L, block, det)
                                                 : slice_step_blocks_per_cu(net->d.L, block, det);
         li.persistent = alg == G4S_ALG_SKL;
     } else {
@@ -569,6 +605,7 @@ void record_launch(g4s_net *net, int alg, bool slice, int block, int grid, bool 
         li.waves_per_simd = r->occupancy;
         li.lds_bytes += r->lds;
     }
+    return li;
 }
 
 // Capture range of a profiler run with --selected-regions, closed on every
@@ -665,7 +702,7 @@ int run_impl(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, 
     if (alg == G4S_ALG_SKL) {
         // every workgroup must be resident: P slices within what the device
         // holds at once of the kernel instantiation that will run
-        const int per_cu = skl_slice_blocks_per_cu(d.L, sblock, det);
+        const int per_cu = skl_resident(net, d.L, sblock, det);
         const int cap = per_cu * std::max(net->cus, 1);
         if (per_cu <= 0 || cap < d.P)
             return fail(G4S_ERR_UNSUPPORTED, "SKL needs %d resident workgroups of %d threads, device holds %d", d.P,
@@ -747,7 +784,7 @@ int run_impl(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, 
     // consume the oldest enqueued chunk
     auto finish = [&](const Pending &pc) -> int {
         HIP_OK(hipEventSynchronize(net->done_ev[pc.buf]));
-        if (net->h_err[pc.buf]) return fail(G4S_ERR_TIMEOUT, "a bounded device wait expired at chunk starting step %d", pc.t0);
+        if (net->h_ctl[pc.buf][0]) return fail(G4S_ERR_TIMEOUT, "a bounded device wait expired at chunk starting step %d", pc.t0);
         float ms = 0.0f;
         HIP_OK(hipEventElapsedTime(&ms, net->ev0[pc.buf], net->ev1[pc.buf]));
         dev_ms += ms;
@@ -755,7 +792,7 @@ int run_impl(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, 
             kern_launches += net->kev_used / 2;
             HIP_OK(kev_collect(net, &kern_ms));
         }
-        const int32_t *hc = net->h_ctl[pc.buf];
+        const int32_t *hc = net->h_ctl[pc.buf] + kCtlHead;
         int64_t mx = 0;
         auto stamp = [&](int r) {
             return (uint64_t)(uint32_t)hc[r * kCtlStride + kCtlStamp] |
@@ -792,8 +829,7 @@ int run_impl(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, 
         const int t0 = (int)(net->t);
         const int buf = chunk_no & 1;
         if (slice_steps && use_graph && (rc = prepare_ab_graph(net, cs, sblock, det))) return abandon(rc);
-        HIP_OK(launch_fill_u32((uint32_t *)d.ctl, (size_t)(cs + 1) * kCtlStride, 0u, net->stream));
-        HIP_OK(launch_fill_u32(d.bar, kBarWords, 0u, net->stream));
+        HIP_OK(launch_fill2_u32((uint32_t *)d.ctl, (size_t)(cs + 1) * kCtlStride, d.bar, kBarWords, 0u, net->stream));
         if (na) d.raster = na->raster ? na->raster + (size_t)c0 * (size_t)words : nullptr;
         if (slice_steps) {
             int32_t *ti = net->h_tinfo + 2 * buf;
@@ -823,14 +859,16 @@ int run_impl(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, 
             }
         }
         HIP_OK(hipEventRecord(net->ev1[buf], net->stream));
-        HIP_OK(launch_stamp(d.ctl, cs, net->stream));
-        if (slice) HIP_OK(launch_reduce_counts(d, cs, net->stream));
+        if (slice)
+            HIP_OK(launch_reduce_counts(d, cs, net->stream));  // stamps the chunk's end too
+        else
+            HIP_OK(launch_stamp(d.ctl, cs, net->stream));
         if (na && na->counts) HIP_OK(launch_copy_counts(d.ctl, na->counts + c0, cs, net->stream));
-        HIP_OK(hipMemcpyAsync(net->h_ctl[buf], d.ctl, (size_t)(cs + 1) * kCtlStride * sizeof(int32_t),
+        // one copy: the error flag, then the chunk's control rows
+        HIP_OK(hipMemcpyAsync(net->h_ctl[buf], d.err, ((size_t)kCtlHead + (size_t)(cs + 1) * kCtlStride) * sizeof(int32_t),
                               hipMemcpyDeviceToHost, net->stream));
         if (raster || fn)
             HIP_OK(hipMemcpyAsync(net->h_raster[buf], d.raster, (size_t)cs * words * sizeof(uint32_t), hipMemcpyDeviceToHost, net->stream));
-        HIP_OK(hipMemcpyAsync(&net->h_err[buf], d.err, sizeof(int32_t), hipMemcpyDeviceToHost, net->stream));
         HIP_OK(hipEventRecord(net->done_ev[buf], net->stream));
         net->t += cs;
         pend[npend++] = Pending{c0, cs, t0, buf};
@@ -911,10 +949,10 @@ int g4s_get_step_times(const g4s_net *net, float *ms, int32_t n)
     return (int)k;
 }
 
-int g4s_get_launch_info(const g4s_net *net, g4s_launch_info *out)
+int g4s_get_launch_info(g4s_net *net, g4s_launch_info *out)
 {
     if (!net || !out) return fail(G4S_ERR_ARG, "NULL argument");
-    *out = net->linfo;
+    *out = launch_report(net);
     return G4S_OK;
 }
 
@@ -983,7 +1021,6 @@ int g4s_destroy(g4s_net *net)
     drop_graph(net);
     for (void *p : net->allocs) (void)hipFree(p);
     if (net->h_tinfo) (void)hipHostFree(net->h_tinfo);
-    if (net->h_err) (void)hipHostFree(net->h_err);
     for (int k = 0; k < 2; ++k) {
         if (net->h_raster[k]) (void)hipHostFree(net->h_raster[k]);
         if (net->h_ctl[k]) (void)hipHostFree(net->h_ctl[k]);

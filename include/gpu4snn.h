@@ -163,7 +163,7 @@ typedef struct g4s_launch_info {
     int32_t persistent;       /* 1: its workgroups wait on each other, so the grid is at
                                  most resident_per_cu * cus                               */
 } g4s_launch_info;
-int g4s_get_launch_info(const g4s_net *net, g4s_launch_info *out);
+int g4s_get_launch_info(g4s_net *net, g4s_launch_info *out);
 
 /* Host-only: the compiler's resource report of one kernel instantiation of
  * this build (name "k_skl_slice", template arguments "1,false,false"; "" for

@@ -352,3 +352,17 @@ def test_large_slices_both_modes(g4s, oracle, block):
                 d = g4s.raster_to_dense(words[:10], n)
                 assert np.array_equal(d, oraster[:10]), (alg, block, regime)
                 assert abs(int(counts.sum()) - int(ocounts.sum())) <= 0.01 * ocounts.sum(), (alg, block, regime)
+
+
+def test_det_nsyn_above_device_builder(g4s, oracle):
+    """Nsyn above the device builder's 1024 (the table is built on the host
+    and uploaded) through every scheme: det bit-identical to the oracle."""
+    n, nsyn, seed, steps = 3000, 2000, 17, 150
+    onet, oraster, ocounts = oracle_raster(oracle, n, nsyn, steps, seed)
+    for alg in ALGS:
+        with g4s.Network(n, nsyn, seed) as net:
+            words, counts = net.run(steps, alg=alg, accum="det")
+            assert np.array_equal(counts, ocounts), alg
+            assert np.array_equal(dense(g4s, words, n), oraster), alg
+            v, _, _, _ = net.state()
+            assert np.array_equal(v.view(np.uint32), onet.v.view(np.uint32)), alg
