@@ -514,17 +514,41 @@ __device__ __forceinline__ void slice_count(const DevNet &d, int rel, int p, int
 // barrier: every thread reads all wave totals.  The caller guarantees a
 // workgroup barrier between the reads of the previous use of s_tmp and
 // this call.
+// Inclusive scans over the 64 lanes of a wave (lane 0 first) by data-parallel
+// primitives: row shifts by 1, 2, 4 and 8 scan each row of 16 lanes (a lane
+// with no source lane in its row adds the identity 0), then the last lane
+// of row 0 is added to row 1 and of row 2 to row 3 (row broadcast 15), and
+// the last lane of row 1 to rows 2 and 3 (row broadcast 31).  A few cycles
+// per step, where a lane shuffle is an LDS round trip.
+__device__ __forceinline__ int wave_inclusive_sum(int x)
+{
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);  // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);  // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);  // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);  // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15, rows 1 and 3
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31, rows 2 and 3
+    return x;
+}
+__device__ __forceinline__ uint32_t wave_inclusive_max(uint32_t x)
+{
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false));
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false));
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false));
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false));
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false));
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false));
+    return x;
+}
+// The value of the wave's last lane, in every lane.
+__device__ __forceinline__ int wave_last(int x) { return __builtin_amdgcn_readlane(x, kWave - 1); }
+
 __device__ __forceinline__ int block_exclusive_scan(int v, int *s_tmp, int *total)
 {
     const int lane = lane_id();
     const int wave = (int)threadIdx.x / kWave;
     const int nw = (int)blockDim.x / kWave;
-    int incl = v;
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-        const int y = __shfl_up(incl, off);
-        if (lane >= off) incl += y;
-    }
+    const int incl = wave_inclusive_sum(v);
     if (lane == kWave - 1) s_tmp[wave] = incl;
     __syncthreads();
     int pre = 0, tot = 0;
@@ -654,12 +678,7 @@ __device__ __forceinline__ void owner_scan(uint16_t *own, int *s_wave, uint32_t 
         raw[q] = ow[q];
         run = max(run, max(raw[q] & 0xFFFFu, raw[q] >> 16));
     }
-    uint32_t incl = run;
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-        const uint32_t y = (uint32_t)__shfl_up((int)incl, off);
-        if (lane >= off) incl = max(incl, y);
-    }
+    const uint32_t incl = wave_inclusive_max(run);
     uint32_t pre = (uint32_t)__shfl_up((int)incl, 1);
     if (lane == 0) pre = 0;
     if (lane == kWave - 1) s_wave[wave] = (int)incl;
@@ -706,9 +725,12 @@ __device__ __forceinline__ void owner_scan(uint16_t *own, int *s_wave, uint32_t 
 struct NoPre {
     __device__ void operator()() const {}
 };
+// backoff (a persistent caller's register, or null): after a failed fast
+// lookup the next kFastBackoff steps take the general path directly.
+constexpr int kFastBackoff = 32;
 template <bool kDet, int kEPT, int kSPT, bool kOwner, bool kCoh = false, class Net, class Pre = NoPre>
 __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int lo, int len, float *dyn,
-                                                 Pre pre = Pre())
+                                                 Pre pre = Pre(), int *backoff = nullptr)
 {
     const int B = (int)blockDim.x;
     const int tid = (int)threadIdx.x;
@@ -762,15 +784,119 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
 #pragma unroll
         for (int k = 0; k < kKeep; ++k) mine += __popc(x[k]);
     }
-    int total = 0;
-    const int first = block_exclusive_scan(mine, scratch, &total);
     const uint32_t stride = (uint32_t)d.P + 1u;
     // targets owned by this thread in the ordered phases: a contiguous run
     const int tper = (len + B - 1) / B;
     const int tb = min(tid * tper, len), te = min(tb + tper, len);
+    int total = 0, first = 0;
+    uint32_t sid[kSPT], e0[kSPT];
+    int lens[kSPT], head[kSPT];
+    int m = 0, E = 0;
+    // Fast lookup (search plans whose every bitmap word sits in registers):
+    // each wave compacts its own spikes into a private part of the ids array
+    // and its lanes load their segment bounds at once; one workgroup pass
+    // over the waves' (spikes, entries) totals then places every spike.  Two
+    // workgroup barriers before the entries instead of four, and the offset
+    // loads go out before the first of them.  A wave with more than 64*kSPT
+    // spikes sends the whole workgroup down the general path below (at the
+    // cost of the two barriers); a persistent caller then skips the attempt
+    // for kFastBackoff steps, since busy steps come in runs.
+    constexpr bool kFast = !kOwner && kWave * kSPT < 256;
+    bool fast = false;
+    if constexpr (kFast) {
+        const bool attempt = W <= B * kKeep && (backoff == nullptr || *backoff == 0);  // uniform
+        if (backoff != nullptr && *backoff > 0) --*backoff;
+        if (attempt) {
+            const int lane = lane_id(), wave = tid / kWave, nw = B / kWave;
+            const int incl = wave_inclusive_sum(mine);
+            const int wtot = wave_last(incl);
+            constexpr int kWaveCap = kWave * kSPT;
+            const bool wfit = wtot <= kWaveCap;  // uniform over the wave
+            uint32_t *wids = ids + wave * kWaveCap;
+            if (wfit && mine > 0) {
+                int idx = incl - mine;
+#pragma unroll
+                for (int k = 0; k < kKeep; ++k) {
+                    uint32_t bits = kept[k];
+                    while (bits) {
+                        wids[idx++] = (uint32_t)(wb + k) * 32u + (uint32_t)__builtin_ctz(bits);
+                        bits &= bits - 1u;
+                    }
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // lane l takes the wave's spikes l, l + 64, ...; in a wave with
+            // spikes that fits, a lane without a spike loads row 0's bounds
+            // (in range) and keeps length 0, so that every load of the wave
+            // issues before the first wait; other waves load nothing
+            uint32_t e1[kSPT];
+            bool has[kSPT];
+#pragma unroll
+            for (int q = 0; q < kSPT; ++q) {
+                has[q] = wfit && lane + kWave * q < wtot;
+                sid[q] = has[q] ? wids[lane + kWave * q] : 0u;
+                e0[q] = e1[q] = 0u;
+            }
+            if (wfit && wtot > 0) {  // uniform over the wave
+#pragma unroll
+                for (int q = 0; q < kSPT; ++q) {
+                    const auto o = d.soff + ((uint64_t)sid[q] * stride + (uint32_t)p);
+                    e0[q] = o[0];
+                    e1[q] = o[1];
+                }
+            }
+            int runq[kSPT];
+            int wlen = 0;
+#pragma unroll
+            for (int q = 0; q < kSPT; ++q) {
+                lens[q] = has[q] ? (int)(e1[q] - e0[q]) : 0;
+                const int inc = wave_inclusive_sum(lens[q]);
+                runq[q] = wlen + inc - lens[q];  // entries before this spike within the wave
+                wlen += wave_last(inc);
+            }
+            // per wave: entries (23 bits) and spikes (8 bits), or the top
+            // bit and the spikes (31 bits) when the wave does not fit
+            const bool packs = wfit && wlen < (1 << 23);
+            if (lane == 0) scratch[wave] = packs ? (int)(((uint32_t)wlen << 8) | (uint32_t)wtot) : -1;
+            __syncthreads();
+            bool all_fit = true;
+            int sp_before = 0, en_before = 0, sp_all = 0, en_all = 0;
+            for (int w = 0; w < nw; ++w) {  // uniform
+                const uint32_t x = (uint32_t)scratch[w];
+                if (x == 0xFFFFFFFFu) all_fit = false;
+                const int sp = (int)(x & 0xFFu), en = (int)(x >> 8);
+                sp_before += w < wave ? sp : 0;
+                en_before += w < wave ? en : 0;
+                sp_all += sp;
+                en_all += en;
+            }
+            if (!all_fit && backoff != nullptr) *backoff = kFastBackoff;
+            if (all_fit) {
+                fast = true;
+                total = m = sp_all;
+                E = en_all;
+                // base[i] overwrites ids[2i..2i+1]: every id was read before
+                // the barrier above
+#pragma unroll
+                for (int q = 0; q < kSPT; ++q) {
+                    head[q] = -1;
+                    const int j = lane + kWave * q;
+                    if (j < wtot) {
+                        const int i = sp_before + j, run = en_before + runq[q];
+                        base[i] = (uint64_t)sid[q] * (uint64_t)d.nsyn + (uint64_t)e0[q] - (uint64_t)run;
+                        segp[i] = run;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (!fast) first = block_exclusive_scan(mine, scratch, &total);
     // 1. spike ids of the first pass, ascending, from the kept words; later
     // passes read the bitmap again
-    if (mine > 0 && first < cap) {
+    if (!fast && mine > 0 && first < cap) {
         int idx = first;
 #pragma unroll
         for (int k = 0; k < kKeep; ++k) {
@@ -797,6 +923,7 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
         }
     }
     for (int pass = 0; pass < total; pass += cap) {
+        if (!fast) {  // the general path places the pass's spikes
         if (pass > 0 && mine > 0 && first < pass + cap && first + mine > pass) {
             int idx = first;
             for (int w0 = wb; w0 < we && idx < pass + cap; w0 += kKeep) {
@@ -816,10 +943,8 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
             }
         }
         __syncthreads();
-        const int m = min(cap, total - pass);
+        m = min(cap, total - pass);
         // 2. segment bounds, thread per spike
-        uint32_t sid[kSPT], e0[kSPT];
-        int lens[kSPT], head[kSPT];
         int lsum = 0;
         if (tid * kSPT >= m) {
 #pragma unroll
@@ -845,7 +970,7 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
 #pragma unroll
         for (int k = 0; k < kSPT; ++k) lsum += lens[k];
         if (kOwner) clear_owner<kEPT>(own0);  // the first window's map, before the scan's barriers
-        int E = 0;
+        E = 0;
         int run = block_exclusive_scan(lsum, scratch, &E);
         // base[i] overwrites ids[2i..2i+1]: every id was read before the scan
 #pragma unroll
@@ -860,6 +985,7 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
             run += lens[k];
         }
         if (!kOwner) __syncthreads();
+        }  // general path
         // 3. entries, thread per synapse, windows of Wd entries with all of a
         // window's loads in flight.  Lanes take consecutive entries
         // (f = f0 + k*B + tid): the loads coalesce, and a target's
@@ -1302,6 +1428,7 @@ __global__ void __launch_bounds__(1024) k_skl_slice(DevNet d, int t0, int steps,
         }
     }
     uint32_t epoch = 0;
+    int backoff = 0;  // slice_accumulate's fast-lookup backoff, uniform over the workgroup
     for (int rel = 0; rel < steps; ++rel) {
         const int t = t0 + rel;
         if (blockIdx.x == 0 && tid == 0) stamp_step(d.ctl, rel);
@@ -1320,9 +1447,9 @@ __global__ void __launch_bounds__(1024) k_skl_slice(DevNet d, int t0, int steps,
                     if (k < ntile && off < len) I0[k] = g[k] * thalamic(d.seed, (uint32_t)t, (uint32_t)(lo + off));
                 }
             };
-            slice_accumulate<kDet, Plan::kEPT, Plan::kSPT, Plan::kOwner, true>(d, t, p, lo, len, dyn, draws);
+            slice_accumulate<kDet, Plan::kEPT, Plan::kSPT, Plan::kOwner, true>(d, t, p, lo, len, dyn, draws, &backoff);
         } else {
-            slice_accumulate<kDet, Plan::kEPT, Plan::kSPT, Plan::kOwner, true>(d, t, p, lo, len, dyn);
+            slice_accumulate<kDet, Plan::kEPT, Plan::kSPT, Plan::kOwner, true>(d, t, p, lo, len, dyn, NoPre(), &backoff);
         }
         uint64_t masks[R];
 #pragma unroll
