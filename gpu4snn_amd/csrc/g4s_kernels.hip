@@ -189,9 +189,10 @@ __device__ __forceinline__ void push_flat(const DevNet &d, int t, int count, int
 // one address are serialised, so counters in separate lines each take a
 // share of the arrivals.  Counters only increase; at epoch e group g is
 // complete when its counter reaches e * (number of workgroups in g).  Lanes
-// 0..kBarGroups-1 of the first wave poll the counters together.  Bounded: returns false (and
-// raises the error flag) if the wait exceeds kSpinLimit or another
-// workgroup raised the flag.
+// 0..kBarGroups-1 of the first wave poll the counters together, and lane
+// kBarGroups the error flag in the same round trip.  Bounded: returns false
+// (and raises the error flag) if the wait exceeds kSpinLimit, or returns
+// false once another workgroup has raised the flag.
 // It orders only data written and read with agent-scope atomics (the spike
 // bitmap of the SKL kernel): every wave waits for its own stores to complete
 // before the workgroup arrives, and the loads after the barrier are issued
@@ -199,6 +200,7 @@ __device__ __forceinline__ void push_flat(const DevNet &d, int t, int count, int
 // write-back of the XCD) and no acquire (an L2 invalidate, which would also
 // evict the static synapse rows that the next step reads).  Between its two
 // workgroup barriers a non-polling wave stores the step's spike count.
+static_assert(kBarGroups < kWave, "the barrier's polling wave also reads the error flag");
 __device__ bool grid_sync(const DevNet &d, uint32_t &epoch, const int *s_wcnt, int32_t *cnt_dst)
 {
     __shared__ int s_ok;
@@ -221,15 +223,19 @@ __device__ bool grid_sync(const DevNet &d, uint32_t &epoch, const int *s_wcnt, i
                                    __HIP_MEMORY_SCOPE_AGENT);
         const uint32_t members = G / kBarGroups + ((uint32_t)lane < G % kBarGroups ? 1u : 0u);
         const uint32_t target = epoch * members;
-        const uint32_t *ctr = &d.bar[(lane < kBarGroups ? lane : 0) * kBarStride];
+        // lanes 0..kBarGroups-1 read the counters and lane kBarGroups the
+        // error flag, in one round trip per poll
+        const uint32_t *src = lane < kBarGroups ? &d.bar[lane * kBarStride]
+                                                : (lane == kBarGroups ? (const uint32_t *)d.err : &d.bar[0]);
         int ok = 1;
         uint32_t spins = 0;
         for (;;) {
-            bool reached = true;
-            if (lane < kBarGroups) reached = __hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target;
-            if (__all(reached)) break;
-            if (lane == 0 && error_set(d)) ok = 0;
-            if (!__shfl(ok, 0)) break;
+            const uint32_t x = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (__all(lane >= kBarGroups || x >= target)) break;
+            if (__any(lane == kBarGroups && x != 0u)) {  // another workgroup gave up
+                ok = 0;
+                break;
+            }
             if (++spins > kSpinLimit) {
                 if (lane == 0) raise_error(d);
                 ok = 0;
