@@ -549,22 +549,54 @@ __device__ __forceinline__ uint32_t wave_inclusive_max(uint32_t x)
 // The value of the wave's last lane, in every lane.
 __device__ __forceinline__ int wave_last(int x) { return __builtin_amdgcn_readlane(x, kWave - 1); }
 
+// This thread's wave index, as a wave-uniform value.
+__device__ __forceinline__ int wave_index() { return __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave); }
+
+// Per-wave values vals[0..nw) in LDS (nw = waves of the workgroup, at most
+// 64): lane l reads vals[l] and one wave scan gives, in every lane, the sum
+// over the waves before this one and (*total) over all of them.  A loop
+// over the waves instead costs each wave a few scalar instructions per
+// wave, and the CU's waves share one scalar unit.
+__device__ __forceinline__ int waves_prefix_sum(const int *vals, int *total)
+{
+    const int lane = lane_id();
+    const int nw = (int)blockDim.x / kWave;
+    const int wave = wave_index();
+    const int incl = wave_inclusive_sum(lane < nw ? vals[lane] : 0);
+    *total = __builtin_amdgcn_readlane(incl, nw - 1);
+    const int before = __builtin_amdgcn_readlane(incl, wave > 0 ? wave - 1 : 0);
+    return wave > 0 ? before : 0;
+}
+// The same for a per-wave value already held by lane `wave` (0 elsewhere).
+__device__ __forceinline__ int waves_prefix_sum_of(int lane_value, int *total)
+{
+    const int nw = (int)blockDim.x / kWave;
+    const int wave = wave_index();
+    const int incl = wave_inclusive_sum(lane_value);
+    *total = __builtin_amdgcn_readlane(incl, nw - 1);
+    const int before = __builtin_amdgcn_readlane(incl, wave > 0 ? wave - 1 : 0);
+    return wave > 0 ? before : 0;
+}
+// The same for the maximum (values >= 0).
+__device__ __forceinline__ uint32_t waves_prefix_max(const int *vals, uint32_t *total)
+{
+    const int lane = lane_id();
+    const int nw = (int)blockDim.x / kWave;
+    const int wave = wave_index();
+    const uint32_t incl = wave_inclusive_max(lane < nw ? (uint32_t)vals[lane] : 0u);
+    *total = (uint32_t)__builtin_amdgcn_readlane((int)incl, nw - 1);
+    const uint32_t before = (uint32_t)__builtin_amdgcn_readlane((int)incl, wave > 0 ? wave - 1 : 0);
+    return wave > 0 ? before : 0u;
+}
+
 __device__ __forceinline__ int block_exclusive_scan(int v, int *s_tmp, int *total)
 {
     const int lane = lane_id();
     const int wave = (int)threadIdx.x / kWave;
-    const int nw = (int)blockDim.x / kWave;
     const int incl = wave_inclusive_sum(v);
     if (lane == kWave - 1) s_tmp[wave] = incl;
     __syncthreads();
-    int pre = 0, tot = 0;
-    for (int w = 0; w < nw; ++w) {  // uniform
-        const int c = s_tmp[w];
-        pre += w < wave ? c : 0;
-        tot += c;
-    }
-    *total = tot;
-    return pre + incl - v;
+    return waves_prefix_sum(s_tmp, total) + incl - v;
 }
 
 // Spikes per pass per thread and entry loads in flight per thread:
@@ -689,13 +721,9 @@ __device__ __forceinline__ void owner_scan(uint16_t *own, int *s_wave, uint32_t 
     if (lane == 0) pre = 0;
     if (lane == kWave - 1) s_wave[wave] = (int)incl;
     __syncthreads();
-    pre = max(pre, *carry);
-    uint32_t tot = *carry;
-    for (int q = 0; q < nw; ++q) {  // uniform
-        const uint32_t y = (uint32_t)s_wave[q];
-        if (q < wave) pre = max(pre, y);
-        tot = max(tot, y);
-    }
+    uint32_t tot = 0;
+    pre = max(pre, max(*carry, waves_prefix_max(s_wave, &tot)));
+    tot = max(tot, *carry);
 #pragma unroll
     for (int q = 0; q < kEPT / 2; ++q) {
         const uint32_t lo16 = max(pre, raw[q] & 0xFFFFu);
@@ -867,17 +895,13 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
             const bool packs = wfit && wlen < (1 << 23);
             if (lane == 0) scratch[wave] = packs ? (int)(((uint32_t)wlen << 8) | (uint32_t)wtot) : -1;
             __syncthreads();
-            bool all_fit = true;
-            int sp_before = 0, en_before = 0, sp_all = 0, en_all = 0;
-            for (int w = 0; w < nw; ++w) {  // uniform
-                const uint32_t x = (uint32_t)scratch[w];
-                if (x == 0xFFFFFFFFu) all_fit = false;
-                const int sp = (int)(x & 0xFFu), en = (int)(x >> 8);
-                sp_before += w < wave ? sp : 0;
-                en_before += w < wave ? en : 0;
-                sp_all += sp;
-                en_all += en;
-            }
+            // lane l decodes wave l's totals; wave scans place this wave
+            const uint32_t xl = lane < nw ? (uint32_t)scratch[lane] : 0u;
+            const bool over = xl == 0xFFFFFFFFu;
+            const bool all_fit = __ballot(over) == 0;  // uniform
+            int sp_all = 0, en_all = 0;
+            const int sp_before = waves_prefix_sum_of(over ? 0 : (int)(xl & 0xFFu), &sp_all);
+            const int en_before = waves_prefix_sum_of(over ? 0 : (int)(xl >> 8), &en_all);
             if (!all_fit && backoff != nullptr) *backoff = kFastBackoff;
             if (all_fit) {
                 fast = true;
@@ -896,7 +920,11 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
                     }
                 }
             }
-            __syncthreads();
+            // placed spikes are visible to every thread; or, on the general
+            // path, every thread has read the totals before its scan reuses
+            // the scratch ints.  With no spike at all, the closing barrier
+            // below is the only one needed.
+            if (!(all_fit && sp_all == 0)) __syncthreads();  // uniform
         }
     }
     if (!fast) first = block_exclusive_scan(mine, scratch, &total);
