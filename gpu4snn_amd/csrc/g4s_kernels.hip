@@ -460,14 +460,57 @@ __host__ __device__ inline int slice_of_block(int b, int g)
     return x * q + (x < r ? x : r) + b / 8;
 }
 
-// Bitmap word w; kCoh: an agent-scope atomic load.
-template <bool kCoh, class P>
-__device__ __forceinline__ uint32_t bm_word(P bm, int w)
+// Bitmaps are read in groups of four words (16 bytes, one load) through a
+// raw buffer resource: dword 3 of a gfx9-family buffer descriptor with a
+// 32-bit data format and no index stride, and the buffer's extent rounded
+// up to a whole group (bitmap allocations are padded to it).  Loads past
+// the extent read 0.  kSc1 is the load's agent-scope coherence bit (sc1),
+// the bit an agent-scope relaxed atomic load carries on gfx950.
+constexpr int kBufWord3 = 0x00020000;
+constexpr int kSc1 = 16;
+
+template <class P>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t bm_rsrc(P bm, int W)
 {
-    if constexpr (kCoh)
-        return __hip_atomic_load(&bm[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    else
-        return bm[w];
+    return __builtin_amdgcn_make_buffer_rsrc((void *)bm, (short)0, ((W + 3) & ~3) * 4, kBufWord3);
+}
+
+// Group g of a bitmap of W words: words 4g .. 4g+3, those at or past W read
+// as 0.  kCoh: the words were stored by other workgroups of the same launch
+// with agent-scope atomics, and the load is coherent at agent scope.
+template <bool kCoh>
+__device__ __forceinline__ uint4 bm_group(__amdgpu_buffer_rsrc_t r, int g, int W)
+{
+    const auto x = __builtin_amdgcn_raw_buffer_load_b128(r, g * 16, 0, kCoh ? kSc1 : 0);
+    uint4 y = make_uint4(x[0], x[1], x[2], x[3]);
+    const int over = 4 * g + 4 - W;  // words of the group at or past W
+    if (over > 0) {
+        y.w = 0u;
+        if (over > 1) y.z = 0u;
+        if (over > 2) y.y = 0u;
+        if (over > 3) y.x = 0u;
+    }
+    return y;
+}
+
+__device__ __forceinline__ int popc4(uint4 x) { return __popc(x.x) + __popc(x.y) + __popc(x.z) + __popc(x.w); }
+
+// Spike ids of bitmap group g, ascending, numbered idx, idx+1, ... in the
+// pass; those numbered in [lo, hi) go to ids[number - lo] (kAll: all of
+// them are known to be, with lo = 0).
+template <bool kAll>
+__device__ __forceinline__ void put_group(uint4 x, int g, int idx, uint32_t *ids, int lo, int hi)
+{
+    const uint32_t w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        uint32_t bits = w[e];
+        while (bits) {
+            if (kAll || (idx >= lo && idx < hi)) ids[idx - lo] = (uint32_t)(4 * g + e) * 32u + (uint32_t)__builtin_ctz(bits);
+            bits &= bits - 1u;
+            ++idx;
+        }
+    }
 }
 
 // Bitmap and raster words of one wave covering [wave_base, wave_base+64)
@@ -551,6 +594,24 @@ __device__ __forceinline__ int wave_last(int x) { return __builtin_amdgcn_readla
 
 // This thread's wave index, as a wave-uniform value.
 __device__ __forceinline__ int wave_index() { return __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave); }
+
+// Spike ids of two bitmap groups per lane, group ga of every lane of the
+// wave before group ga + 64 of any (the lanes' groups ga are consecutive):
+// the wave's spikes of the two are numbered from `run` (wave-uniform) in
+// ascending id and placed by put_group.  One wave scan over both counts
+// packed in 16-bit halves (at most 128 per group, 8192 per wave: no carry).
+// Returns run advanced past both.
+template <bool kAll>
+__device__ __forceinline__ int put_pair(uint4 xa, uint4 xb, int ga, int run, uint32_t *ids, int lo, int hi)
+{
+    const int pk = popc4(xa) | (popc4(xb) << 16);
+    const int incl = wave_inclusive_sum(pk);
+    const int tot = wave_last(incl);
+    const int ex = incl - pk;
+    put_group<kAll>(xa, ga, run + (ex & 0xFFFF), ids, lo, hi);
+    put_group<kAll>(xb, ga + kWave, run + (tot & 0xFFFF) + (ex >> 16), ids, lo, hi);
+    return run + (tot & 0xFFFF) + (tot >> 16);
+}
 
 // Per-wave values vals[0..nw) in LDS (nw = waves of the workgroup, at most
 // 64): lane l reads vals[l] and one wave scan gives, in every lane, the sum
@@ -737,7 +798,7 @@ __device__ __forceinline__ void owner_scan(uint16_t *own, int *s_wave, uint32_t 
 // acc[0..len) <- input of slice p from the rows that spiked at step t-1.
 // Three dependent global levels, each with all its loads in flight:
 //  1. the step's bitmap, compacted in LDS into the ascending spike list
-//     (thread i owns a contiguous run of words);
+//     (the lanes of a wave own consecutive groups of four words);
 //  2. thread per spike: the row's segment bounds for slice p (offset table),
 //     and a scan of the segment lengths gives each segment's offset in the
 //     concatenation of the pass's segments;
@@ -753,8 +814,8 @@ __device__ __forceinline__ void owner_scan(uint16_t *own, int *s_wave, uint32_t 
 // (source, target) order; each window of them is counting-sorted by target
 // in LDS (count, scan, place) and each target then adds its entries in key
 // order, so the sum never depends on the order of the LDS atomics.
-// kCoh: bitmap words are read with agent-scope atomics (written by other
-// workgroups of the same launch).  pre() is called once, by every thread,
+// kCoh: bitmap words are read with agent-scope coherent loads (written by
+// other workgroups of the same launch).  pre() is called once, by every thread,
 // while the first bitmap loads are in flight.
 struct NoPre {
     __device__ void operator()() const {}
@@ -796,33 +857,38 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
     }
     const auto bm = ((t - 1) & 1) ? d.bm1 : d.bm0;
     const int W = d.words;
-    const int per = (W + B - 1) / B;
-    const int wb = min(tid * per, W), we = min(wb + per, W);
-    // up to kKeep words stay in registers for the id pass below; longer
-    // runs (large networks) are read kKeep words at a time, all of a group's
-    // loads in flight before the first wait
-    constexpr int kKeep = 8;
-    uint32_t kept[kKeep];
-    int mine = 0;
-#pragma unroll
-    for (int k = 0; k < kKeep; ++k) kept[k] = (wb + k < we) ? bm_word<kCoh>(bm, wb + k) : 0u;
+    // Bitmap ownership: groups of four consecutive words, one 16-byte load
+    // each, and the lanes of a wave take consecutive groups, so that one
+    // load instruction of a wave reads 1 KB of consecutive words (eight
+    // whole 128-byte lines; a run of words per thread would touch one line
+    // per lane, and coherent loads are not merged in the caches).  With
+    // G = ceil(W / 4B) groups per thread, lane l of wave v takes groups
+    // g0 + 64k, k < G, g0 = 64*G*v + l: the ascending word order is (wave,
+    // k, lane, word of the group).  The first two groups stay in registers
+    // for the id pass below; longer bitmaps (large networks) are read again
+    // two groups at a time.
+    const int lane = lane_id(), wave = tid / kWave;
+    const int G = (W + 4 * B - 1) / (4 * B);
+    const int g0 = kWave * G * wave + lane;
+    const auto rs = bm_rsrc(bm, W);
+    const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
+    const uint4 kept0 = bm_group<kCoh>(rs, g0, W);
+    const uint4 kept1 = G > 1 ? bm_group<kCoh>(rs, g0 + kWave, W) : zero4;
     // the caller's work that needs no spikes (the neurons' thalamic draws)
     // runs while the bitmap loads are in flight
     pre();
-#pragma unroll
-    for (int k = 0; k < kKeep; ++k) mine += __popc(kept[k]);
-    for (int w0 = wb + kKeep; w0 < we; w0 += kKeep) {
-        uint32_t x[kKeep];
-#pragma unroll
-        for (int k = 0; k < kKeep; ++k) x[k] = (w0 + k < we) ? bm_word<kCoh>(bm, w0 + k) : 0u;
-#pragma unroll
-        for (int k = 0; k < kKeep; ++k) mine += __popc(x[k]);
+    int mine = popc4(kept0) + popc4(kept1);
+    for (int k = 2; k < G; k += 2) {
+        const uint4 xa = bm_group<kCoh>(rs, g0 + kWave * k, W);
+        const uint4 xb = k + 1 < G ? bm_group<kCoh>(rs, g0 + kWave * (k + 1), W) : zero4;
+        mine += popc4(xa) + popc4(xb);
     }
     const uint32_t stride = (uint32_t)d.P + 1u;
     // targets owned by this thread in the ordered phases: a contiguous run
     const int tper = (len + B - 1) / B;
     const int tb = min(tid * tper, len), te = min(tb + tper, len);
-    int total = 0, first = 0;
+    int total = 0;
+    int wrun = 0, wsum = 0;  // general path: spikes before this wave, in it
     uint32_t sid[kSPT], e0[kSPT];
     int lens[kSPT], head[kSPT];
     int m = 0, E = 0;
@@ -838,25 +904,21 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
     constexpr bool kFast = !kOwner && kWave * kSPT < 256;
     bool fast = false;
     if constexpr (kFast) {
-        const bool attempt = W <= B * kKeep && (backoff == nullptr || *backoff == 0);  // uniform
+        const bool attempt = G <= 2 && (backoff == nullptr || *backoff == 0);  // uniform
         if (backoff != nullptr && *backoff > 0) --*backoff;
         if (attempt) {
-            const int lane = lane_id(), wave = tid / kWave, nw = B / kWave;
-            const int incl = wave_inclusive_sum(mine);
-            const int wtot = wave_last(incl);
+            const int nw = B / kWave;
+            const int pk = popc4(kept0) | (popc4(kept1) << 16);
+            const int incl = wave_inclusive_sum(pk);
+            const int tot = wave_last(incl);
+            const int wtot = (tot & 0xFFFF) + (tot >> 16);
             constexpr int kWaveCap = kWave * kSPT;
             const bool wfit = wtot <= kWaveCap;  // uniform over the wave
             uint32_t *wids = ids + wave * kWaveCap;
-            if (wfit && mine > 0) {
-                int idx = incl - mine;
-#pragma unroll
-                for (int k = 0; k < kKeep; ++k) {
-                    uint32_t bits = kept[k];
-                    while (bits) {
-                        wids[idx++] = (uint32_t)(wb + k) * 32u + (uint32_t)__builtin_ctz(bits);
-                        bits &= bits - 1u;
-                    }
-                }
+            if (wfit && mine > 0) {  // both groups' ids, as put_pair places them
+                const int ex = incl - pk;
+                put_group<true>(kept0, g0, ex & 0xFFFF, wids, 0, 0);
+                put_group<true>(kept1, g0 + kWave, (tot & 0xFFFF) + (ex >> 16), wids, 0, 0);
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -927,53 +989,32 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
             if (!(all_fit && sp_all == 0)) __syncthreads();  // uniform
         }
     }
-    if (!fast) first = block_exclusive_scan(mine, scratch, &total);
-    // 1. spike ids of the first pass, ascending, from the kept words; later
-    // passes read the bitmap again
-    if (!fast && mine > 0 && first < cap) {
-        int idx = first;
-#pragma unroll
-        for (int k = 0; k < kKeep; ++k) {
-            uint32_t bits = kept[k];
-            while (bits) {
-                if (idx < cap) ids[idx] = (uint32_t)(wb + k) * 32u + (uint32_t)__builtin_ctz(bits);
-                bits &= bits - 1u;
-                ++idx;
-            }
-        }
-        for (int w0 = wb + kKeep; w0 < we && idx < cap; w0 += kKeep) {
-            uint32_t x[kKeep];
-#pragma unroll
-            for (int k = 0; k < kKeep; ++k) x[k] = (w0 + k < we) ? bm_word<kCoh>(bm, w0 + k) : 0u;
-#pragma unroll
-            for (int k = 0; k < kKeep; ++k) {
-                uint32_t bits = x[k];
-                while (bits) {
-                    if (idx < cap) ids[idx] = (uint32_t)(w0 + k) * 32u + (uint32_t)__builtin_ctz(bits);
-                    bits &= bits - 1u;
-                    ++idx;
-                }
-            }
+    if (!fast) {
+        // spikes before this wave: a scan over the waves' totals
+        wsum = wave_last(wave_inclusive_sum(mine));
+        if (lane == kWave - 1) scratch[wave] = wsum;
+        __syncthreads();
+        wrun = waves_prefix_sum(scratch, &total);
+    }
+    // 1. spike ids of the first pass, ascending, from the kept groups; later
+    // groups and later passes read the bitmap again.  The conditions are
+    // uniform over the wave (put_pair scans it).
+    if (!fast && wsum > 0 && wrun < cap) {
+        int run = put_pair<false>(kept0, kept1, g0, wrun, ids, 0, cap);
+        for (int k = 2; k < G && run < cap; k += 2) {
+            const uint4 xa = bm_group<kCoh>(rs, g0 + kWave * k, W);
+            const uint4 xb = k + 1 < G ? bm_group<kCoh>(rs, g0 + kWave * (k + 1), W) : zero4;
+            run = put_pair<false>(xa, xb, g0 + kWave * k, run, ids, 0, cap);
         }
     }
     for (int pass = 0; pass < total; pass += cap) {
         if (!fast) {  // the general path places the pass's spikes
-        if (pass > 0 && mine > 0 && first < pass + cap && first + mine > pass) {
-            int idx = first;
-            for (int w0 = wb; w0 < we && idx < pass + cap; w0 += kKeep) {
-                uint32_t x[kKeep];
-#pragma unroll
-                for (int k = 0; k < kKeep; ++k) x[k] = (w0 + k < we) ? bm_word<kCoh>(bm, w0 + k) : 0u;
-#pragma unroll
-                for (int k = 0; k < kKeep; ++k) {
-                    uint32_t bits = x[k];
-                    while (bits) {
-                        if (idx >= pass && idx < pass + cap)
-                            ids[idx - pass] = (uint32_t)(w0 + k) * 32u + (uint32_t)__builtin_ctz(bits);
-                        bits &= bits - 1u;
-                        ++idx;
-                    }
-                }
+        if (pass > 0 && wsum > 0 && wrun < pass + cap && wrun + wsum > pass) {  // uniform over the wave
+            int run = wrun;
+            for (int k = 0; k < G && run < pass + cap; k += 2) {
+                const uint4 xa = bm_group<kCoh>(rs, g0 + kWave * k, W);
+                const uint4 xb = k + 1 < G ? bm_group<kCoh>(rs, g0 + kWave * (k + 1), W) : zero4;
+                run = put_pair<false>(xa, xb, g0 + kWave * k, run, ids, pass, pass + cap);
             }
         }
         __syncthreads();

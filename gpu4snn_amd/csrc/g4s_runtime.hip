@@ -476,8 +476,11 @@ int g4s_create(const g4s_spec *spec, g4s_net **out)
     for (float **f : fields)
         if ((rc = net->alloc(f, n))) return bail(rc);
     if ((rc = net->alloc(&d.nospks, n))) return bail(rc);
-    if ((rc = net->alloc(&d.bm0, (size_t)d.words))) return bail(rc);
-    if ((rc = net->alloc(&d.bm1, (size_t)d.words))) return bail(rc);
+    // bitmaps padded to whole groups of four words: the slice kernels read
+    // them 16 bytes at a time
+    const size_t bm_words = ((size_t)d.words + 3) & ~(size_t)3;
+    if ((rc = net->alloc(&d.bm0, bm_words))) return bail(rc);
+    if ((rc = net->alloc(&d.bm1, bm_words))) return bail(rc);
     if ((rc = net->alloc(&d.q0, n))) return bail(rc);
     if ((rc = net->alloc(&d.q1, n))) return bail(rc);
     if ((rc = net->alloc(&d.bar, kBarWords))) return bail(rc);
