@@ -701,6 +701,15 @@ constexpr bool kOwnerSingle = false, kOwnerBatch = true;
 //   search lookup: segment offsets int x cap, padded to 16 bytes
 //   atomic, owner lookup: own u16 x Wd, twice (maps of even and odd windows)
 //   det:    w f32 x Wd | keys u16 x Wd (the owner map too) | cursor, start int x L
+//   atomic, wave-local lookup (search plans with at most 64*kSPT < 256
+//           spikes per wave): a map u8 x kWaveMap per wave after the search
+//           lookup's offsets; the offsets region holds each wave's spike ids
+//           and the base region each wave's segment bases
+constexpr int kWaveMap = 512;  // entries per wave-local chunk: 8 per lane
+__host__ __device__ constexpr bool wave_local_plan(bool det, int spt, bool owner)
+{
+    return !det && !owner && 64 * spt < 256;
+}
 __host__ __device__ inline int acc_floats(int L) { return (L + 3) & ~3; }
 __host__ __device__ inline size_t scan_bytes(int block) { return ((size_t)(block / 64 + 1) * 4 + 15) & ~(size_t)15; }
 __host__ __device__ inline size_t slice_lds_bytes(int L, int block, bool det, int ept, int spt, bool owner)
@@ -709,6 +718,7 @@ __host__ __device__ inline size_t slice_lds_bytes(int L, int block, bool det, in
     size_t b = (size_t)acc_floats(L) * 4 + cap * 8 + scan_bytes(block);
     if (!owner) b += (cap * 4 + 15) & ~(size_t)15;
     if (det) return b + wd * 6 + (size_t)L * 8;
+    if (wave_local_plan(det, spt, owner)) b += (size_t)(block / 64) * kWaveMap;
     return b + (owner ? wd * 4 : 0);
 }
 
@@ -795,6 +805,123 @@ __device__ __forceinline__ void owner_scan(uint16_t *own, int *s_wave, uint32_t 
     __syncthreads();
 }
 
+// acc[e.x - lo] += weight of e for each entry e of x (target 0xFFFFFFFF:
+// none): binary32 adds applied atomically, in any order, as compare-and-swap
+// loops on the bit patterns.  Every first attempt of the thread is issued
+// before any result is looked at, and a lost race retries from the value it
+// saw.  On MI355X this runs the batched step 1.5x faster than the LDS float
+// atomic add (ds_add_f32).
+template <int K>
+__device__ __forceinline__ void lds_add_cas(float *acc, const uint2 (&x)[K], int lo)
+{
+    uint32_t *ai = (uint32_t *)acc;
+    uint32_t seen[K], got[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) seen[k] = x[k].x != 0xFFFFFFFFu ? ai[(int)x[k].x - lo] : 0u;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        got[k] = seen[k];
+        if (x[k].x != 0xFFFFFFFFu)
+            got[k] = atomicCAS(&ai[(int)x[k].x - lo], seen[k], __float_as_uint(__uint_as_float(seen[k]) + u2f(x[k].y)));
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        uint32_t cur = got[k];
+        while (cur != seen[k]) {  // lost a race: at least one other add landed
+            seen[k] = cur;
+            cur = atomicCAS(&ai[(int)x[k].x - lo], seen[k], __float_as_uint(__uint_as_float(seen[k]) + u2f(x[k].y)));
+        }
+    }
+}
+
+// LDS accesses of the other lanes of this wave that precede the call in
+// program order are visible to the accesses that follow it.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Atomic-mode accumulation of the spikes of one wave's bitmap groups (its
+// lane's groups g0 and g0 + 64 hold kept0 and kept1), with no exchange
+// between the waves of the workgroup: the adds may land in any order, so
+// each wave takes the entries of its own spikes.  Per batch of up to
+// 64*kSPT of the wave's spikes (one batch unless the wave holds more):
+//  1. the spike ids, ascending, into the wave's id array (wids);
+//  2. lane per spike: the row's segment bounds for slice p, a wave scan of
+//     the lengths numbering the concatenation of the batch's segments, and
+//     per spike the table index of its entry 0 of that numbering (wbs);
+//  3. per chunk of kWaveMap entries: every spike writes its index into the
+//     map at the positions of its segment, then lane l takes entries
+//     c0 + 64i + l: one map read, one base read, one table load each.
+// Only wave-level ordering is needed (wave_sync); the caller brackets the
+// call with workgroup barriers (acc cleared before, every add done after).
+template <bool kCoh, int kSPT, class Net>
+__device__ __forceinline__ void wave_accumulate(const Net &d, int p, int lo, float *acc, uint4 kept0, uint4 kept1,
+                                                int g0, uint32_t *wids, uint64_t *wbs, uint8_t *wmap)
+{
+    constexpr int kWaveCap = kWave * kSPT;
+    constexpr int kEPTw = kWaveMap / kWave;
+    static_assert(kWaveCap <= 256, "wave-local spike indices are bytes");
+    const int lane = lane_id();
+    const uint32_t stride = (uint32_t)d.P + 1u;
+    const int pk = popc4(kept0) | (popc4(kept1) << 16);
+    const int incl = wave_inclusive_sum(pk);
+    const int tot = wave_last(incl);
+    const int T0 = tot & 0xFFFF;
+    const int wtot = T0 + (tot >> 16);
+    const int ex = incl - pk;
+    const int i0 = ex & 0xFFFF, i1 = T0 + (ex >> 16);
+    for (int b0 = 0; b0 < wtot; b0 += kWaveCap) {  // uniform over the wave
+        put_group<false>(kept0, g0, i0, wids, b0, b0 + kWaveCap);
+        put_group<false>(kept1, g0 + kWave, i1, wids, b0, b0 + kWaveCap);
+        wave_sync();
+        const int nb = min(kWaveCap, wtot - b0);
+        // a lane without a spike loads row 0's bounds (in range) and keeps
+        // length 0, so that every bound load issues before the first wait
+        uint32_t sid[kSPT], a0[kSPT], a1[kSPT];
+#pragma unroll
+        for (int q = 0; q < kSPT; ++q) sid[q] = lane + kWave * q < nb ? wids[lane + kWave * q] : 0u;
+#pragma unroll
+        for (int q = 0; q < kSPT; ++q) {
+            const auto o = d.soff + ((uint64_t)sid[q] * stride + (uint32_t)p);
+            a0[q] = o[0];
+            a1[q] = o[1];
+        }
+        int len[kSPT], run[kSPT];
+        int wlen = 0;
+#pragma unroll
+        for (int q = 0; q < kSPT; ++q) {
+            len[q] = lane + kWave * q < nb ? (int)(a1[q] - a0[q]) : 0;
+            const int inc = wave_inclusive_sum(len[q]);
+            run[q] = wlen + inc - len[q];
+            wlen += wave_last(inc);
+        }
+#pragma unroll
+        for (int q = 0; q < kSPT; ++q)
+            if (len[q] > 0)
+                wbs[lane + kWave * q] = (uint64_t)sid[q] * (uint64_t)d.nsyn + (uint64_t)a0[q] - (uint64_t)run[q];
+        for (int c0 = 0; c0 < wlen; c0 += kWaveMap) {  // uniform over the wave
+#pragma unroll
+            for (int q = 0; q < kSPT; ++q) {
+                const int s = max(run[q], c0), e = min(run[q] + len[q], c0 + kWaveMap);
+                for (int f = s; f < e; ++f) wmap[f - c0] = (uint8_t)(lane + kWave * q);
+            }
+            wave_sync();
+            uint2 x[kEPTw];
+#pragma unroll
+            for (int i = 0; i < kEPTw; ++i) {
+                const int f = c0 + kWave * i + lane;
+                x[i] = make_uint2(0xFFFFFFFFu, 0u);
+                if (f < wlen) x[i] = entry_at(d.syn, (size_t)(wbs[wmap[f - c0]] + (uint64_t)f));
+            }
+            lds_add_cas<kEPTw>(acc, x, lo);
+            wave_sync();  // the map, ids and bases are rewritten next
+        }
+    }
+}
+
 // acc[0..len) <- input of slice p from the rows that spiked at step t-1.
 // Three dependent global levels, each with all its loads in flight:
 //  1. the step's bitmap, compacted in LDS into the ascending spike list
@@ -859,28 +986,42 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
     const int W = d.words;
     // Bitmap ownership: groups of four consecutive words, one 16-byte load
     // each, and the lanes of a wave take consecutive groups, so that one
-    // load instruction of a wave reads 1 KB of consecutive words (eight
-    // whole 128-byte lines; a run of words per thread would touch one line
-    // per lane, and coherent loads are not merged in the caches).  With
-    // G = ceil(W / 4B) groups per thread, lane l of wave v takes groups
-    // g0 + 64k, k < G, g0 = 64*G*v + l: the ascending word order is (wave,
-    // k, lane, word of the group).  The first two groups stay in registers
-    // for the id pass below; longer bitmaps (large networks) are read again
-    // two groups at a time.
+    // load instruction of a wave reads up to 1 KB of consecutive words
+    // (whole 128-byte lines; a run of words per thread would touch one line
+    // per lane, and coherent loads are not merged in the caches).  The
+    // groups are spread evenly over the waves: wave v owns Gw = ceil(groups
+    // / waves) of them from v*Gw, and its lane l takes groups v*Gw + 64k +
+    // l for 64k + l < Gw, k < G = ceil(Gw / 64).  The ascending word order
+    // is (wave, k, lane, word of the group).  The first two groups of a
+    // lane stay in registers for the id pass below; longer bitmaps (large
+    // networks) are read again two groups at a time.
     const int lane = lane_id(), wave = tid / kWave;
-    const int G = (W + 4 * B - 1) / (4 * B);
-    const int g0 = kWave * G * wave + lane;
+    const int nwaves = B / kWave;
+    const int Gw = ((W + 3) / 4 + nwaves - 1) / nwaves;
+    const int G = (Gw + kWave - 1) / kWave;
+    const int g0 = Gw * wave + lane;
     const auto rs = bm_rsrc(bm, W);
     const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
-    const uint4 kept0 = bm_group<kCoh>(rs, g0, W);
-    const uint4 kept1 = G > 1 ? bm_group<kCoh>(rs, g0 + kWave, W) : zero4;
+    auto group = [&](int k) { return kWave * k + lane < Gw ? bm_group<kCoh>(rs, g0 + kWave * k, W) : zero4; };
+    const uint4 kept0 = group(0);
+    const uint4 kept1 = group(1);
     // the caller's work that needs no spikes (the neurons' thalamic draws)
     // runs while the bitmap loads are in flight
     pre();
+    constexpr bool kWaveLocal = wave_local_plan(kDet, kSPT, kOwner);
+    if constexpr (kWaveLocal) {
+        if (G <= 2) {  // uniform: every group of the wave is in registers
+            __syncthreads();  // acc is cleared before any wave adds into it
+            uint8_t *wmap = (uint8_t *)(w1 + (((size_t)cap * 4 + 15) & ~(size_t)15)) + (size_t)wave * kWaveMap;
+            wave_accumulate<kCoh, kSPT>(d, p, lo, acc, kept0, kept1, g0, (uint32_t *)segp + wave * kWave * kSPT,
+                                        base + wave * kWave * kSPT, wmap);
+            __syncthreads();
+            return;
+        }
+    }
     int mine = popc4(kept0) + popc4(kept1);
     for (int k = 2; k < G; k += 2) {
-        const uint4 xa = bm_group<kCoh>(rs, g0 + kWave * k, W);
-        const uint4 xb = k + 1 < G ? bm_group<kCoh>(rs, g0 + kWave * (k + 1), W) : zero4;
+        const uint4 xa = group(k), xb = group(k + 1);
         mine += popc4(xa) + popc4(xb);
     }
     const uint32_t stride = (uint32_t)d.P + 1u;
@@ -901,13 +1042,13 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
     // spikes sends the whole workgroup down the general path below (at the
     // cost of the two barriers); a persistent caller then skips the attempt
     // for kFastBackoff steps, since busy steps come in runs.
-    constexpr bool kFast = !kOwner && kWave * kSPT < 256;
+    constexpr bool kFast = !kOwner && kWave * kSPT < 256 && !kWaveLocal;
     bool fast = false;
     if constexpr (kFast) {
         const bool attempt = G <= 2 && (backoff == nullptr || *backoff == 0);  // uniform
         if (backoff != nullptr && *backoff > 0) --*backoff;
         if (attempt) {
-            const int nw = B / kWave;
+            const int nw = nwaves;
             const int pk = popc4(kept0) | (popc4(kept1) << 16);
             const int incl = wave_inclusive_sum(pk);
             const int tot = wave_last(incl);
@@ -1002,9 +1143,7 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
     if (!fast && wsum > 0 && wrun < cap) {
         int run = put_pair<false>(kept0, kept1, g0, wrun, ids, 0, cap);
         for (int k = 2; k < G && run < cap; k += 2) {
-            const uint4 xa = bm_group<kCoh>(rs, g0 + kWave * k, W);
-            const uint4 xb = k + 1 < G ? bm_group<kCoh>(rs, g0 + kWave * (k + 1), W) : zero4;
-            run = put_pair<false>(xa, xb, g0 + kWave * k, run, ids, 0, cap);
+            run = put_pair<false>(group(k), group(k + 1), g0 + kWave * k, run, ids, 0, cap);
         }
     }
     for (int pass = 0; pass < total; pass += cap) {
@@ -1012,9 +1151,7 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
         if (pass > 0 && wsum > 0 && wrun < pass + cap && wrun + wsum > pass) {  // uniform over the wave
             int run = wrun;
             for (int k = 0; k < G && run < pass + cap; k += 2) {
-                const uint4 xa = bm_group<kCoh>(rs, g0 + kWave * k, W);
-                const uint4 xb = k + 1 < G ? bm_group<kCoh>(rs, g0 + kWave * (k + 1), W) : zero4;
-                run = put_pair<false>(xa, xb, g0 + kWave * k, run, ids, pass, pass + cap);
+                run = put_pair<false>(group(k), group(k + 1), g0 + kWave * k, run, ids, pass, pass + cap);
             }
         }
         __syncthreads();
@@ -1111,32 +1248,7 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
                 if (f0 + k * B + tid < E) x[k] = entry_at(d.syn, (size_t)idx[k]);
             }
             if (!kDet) {
-                // binary32 adds applied atomically, in any order, as
-                // compare-and-swap loops on the bit patterns: every first
-                // attempt of the thread is issued before any result is
-                // looked at, and a lost race retries from the value it saw.
-                // On MI355X this runs the batched step 1.5x faster than the
-                // LDS float atomic add (ds_add_f32).
-                uint32_t *ai = (uint32_t *)acc;
-                uint32_t seen[kEPT], got[kEPT];
-#pragma unroll
-                for (int k = 0; k < kEPT; ++k) seen[k] = x[k].x != 0xFFFFFFFFu ? ai[(int)x[k].x - lo] : 0u;
-#pragma unroll
-                for (int k = 0; k < kEPT; ++k) {
-                    got[k] = seen[k];
-                    if (x[k].x != 0xFFFFFFFFu)
-                        got[k] = atomicCAS(&ai[(int)x[k].x - lo], seen[k],
-                                           __float_as_uint(__uint_as_float(seen[k]) + u2f(x[k].y)));
-                }
-#pragma unroll
-                for (int k = 0; k < kEPT; ++k) {
-                    uint32_t cur = got[k];
-                    while (cur != seen[k]) {  // lost a race: at least one other add landed
-                        seen[k] = cur;
-                        cur = atomicCAS(&ai[(int)x[k].x - lo], seen[k],
-                                        __float_as_uint(__uint_as_float(seen[k]) + u2f(x[k].y)));
-                    }
-                }
+                lds_add_cas<kEPT>(acc, x, lo);
             } else {
                 // ordered: counting sort of the window by target, then each
                 // target adds its entries in window-offset (= source) order
