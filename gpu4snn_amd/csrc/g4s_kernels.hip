@@ -705,7 +705,7 @@ constexpr bool kOwnerSingle = false, kOwnerBatch = true;
 //           spikes per wave): a map u8 x kWaveMap per wave after the search
 //           lookup's offsets; the offsets region holds each wave's spike ids
 //           and the base region each wave's segment bases
-constexpr int kWaveMap = 512;  // entries per wave-local chunk: 8 per lane
+constexpr int kWaveMap = 384;  // entries per wave-local chunk: 6 per lane
 __host__ __device__ constexpr bool wave_local_plan(bool det, int spt, bool owner)
 {
     return !det && !owner && 64 * spt < 256;
@@ -843,6 +843,33 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Entries c0 + 64i + lane (i < K, below wlen) of a wave-local chunk: all map
+// reads, then all base reads, then all table loads are issued before the
+// first wait (the reads are unconditional; a slot past wlen reads spike 0's
+// base and loads nothing), then the adds.
+template <int K, class Net>
+__device__ __forceinline__ void wave_chunk(const Net &d, int lo, float *acc, int c0, int wlen, const uint64_t *wbs,
+                                           const uint8_t *wmap)
+{
+    const int lane = lane_id();
+    uint32_t jm[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) jm[i] = wmap[kWave * i + lane];
+    uint64_t idx[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+        const int f = c0 + kWave * i + lane;
+        idx[i] = wbs[f < wlen ? jm[i] : 0u] + (uint64_t)f;
+    }
+    uint2 x[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+        x[i] = make_uint2(0xFFFFFFFFu, 0u);
+        if (c0 + kWave * i + lane < wlen) x[i] = entry_at(d.syn, (size_t)idx[i]);
+    }
+    lds_add_cas<K>(acc, x, lo);
+}
+
 // Atomic-mode accumulation of the spikes of one wave's bitmap groups (its
 // lane's groups g0 and g0 + 64 hold kept0 and kept1), with no exchange
 // between the waves of the workgroup: the adds may land in any order, so
@@ -909,14 +936,10 @@ __device__ __forceinline__ void wave_accumulate(const Net &d, int p, int lo, flo
                 for (int f = s; f < e; ++f) wmap[f - c0] = (uint8_t)(lane + kWave * q);
             }
             wave_sync();
-            uint2 x[kEPTw];
-#pragma unroll
-            for (int i = 0; i < kEPTw; ++i) {
-                const int f = c0 + kWave * i + lane;
-                x[i] = make_uint2(0xFFFFFFFFu, 0u);
-                if (f < wlen) x[i] = entry_at(d.syn, (size_t)(wbs[wmap[f - c0]] + (uint64_t)f));
-            }
-            lds_add_cas<kEPTw>(acc, x, lo);
+            if (wlen - c0 <= kWaveMap / 2)  // uniform: half the slots hold every entry
+                wave_chunk<kEPTw / 2>(d, lo, acc, c0, wlen, wbs, wmap);
+            else
+                wave_chunk<kEPTw>(d, lo, acc, c0, wlen, wbs, wmap);
             wave_sync();  // the map, ids and bases are rewritten next
         }
     }
@@ -942,7 +965,9 @@ __device__ __forceinline__ void wave_accumulate(const Net &d, int p, int lo, flo
 // in LDS (count, scan, place) and each target then adds its entries in key
 // order, so the sum never depends on the order of the LDS atomics.
 // kCoh: bitmap words are read with agent-scope coherent loads (written by
-// other workgroups of the same launch).  pre() is called once, by every thread,
+// other workgroups of the same launch).  kZeroed: acc[0..len) is zero on
+// entry, ordered before the call by a workgroup barrier (the caller clears
+// it as it reads it).  pre() is called once, by every thread,
 // while the first bitmap loads are in flight.
 struct NoPre {
     __device__ void operator()() const {}
@@ -950,7 +975,8 @@ struct NoPre {
 // backoff (a persistent caller's register, or null): after a failed fast
 // lookup the next kFastBackoff steps take the general path directly.
 constexpr int kFastBackoff = 32;
-template <bool kDet, int kEPT, int kSPT, bool kOwner, bool kCoh = false, class Net, class Pre = NoPre>
+template <bool kDet, int kEPT, int kSPT, bool kOwner, bool kCoh = false, bool kZeroed = false, class Net,
+          class Pre = NoPre>
 __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int lo, int len, float *dyn,
                                                  Pre pre = Pre(), int *backoff = nullptr)
 {
@@ -973,10 +999,11 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
     uint16_t *dkey = (uint16_t *)(w2 + (size_t)Wd * 4);  // det: also the owner map
     int *dcur = (int *)(w2 + (size_t)Wd * 6);
     int *dstart = dcur + L;
-    for (int i = tid; i < len; i += B) {
-        acc[i] = 0.0f;
-        if (kDet) dcur[i] = 0;
-    }
+    if (!kZeroed)
+        for (int i = tid; i < len; i += B) {
+            acc[i] = 0.0f;
+            if (kDet) dcur[i] = 0;
+        }
     if (t == 0) {
         pre();
         __syncthreads();
@@ -1011,7 +1038,7 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
     constexpr bool kWaveLocal = wave_local_plan(kDet, kSPT, kOwner);
     if constexpr (kWaveLocal) {
         if (G <= 2) {  // uniform: every group of the wave is in registers
-            __syncthreads();  // acc is cleared before any wave adds into it
+            if (!kZeroed) __syncthreads();  // acc is cleared before any wave adds into it
             uint8_t *wmap = (uint8_t *)(w1 + (((size_t)cap * 4 + 15) & ~(size_t)15)) + (size_t)wave * kWaveMap;
             wave_accumulate<kCoh, kSPT>(d, p, lo, acc, kept0, kept1, g0, (uint32_t *)segp + wave * kWave * kSPT,
                                         base + wave * kWave * kSPT, wmap);
@@ -1616,13 +1643,21 @@ __global__ void __launch_bounds__(1024) k_skl_slice(DevNet d, int t0, int steps,
     }
     uint32_t epoch = 0;
     int backoff = 0;  // slice_accumulate's fast-lookup backoff, uniform over the workgroup
+    // atomic mode: each neuron's input is cleared as the update reads it, so
+    // the accumulation needs no clearing barrier of its own
+    if constexpr (!kDet) {
+        for (int i = tid; i < len; i += B) acc[i] = 0.0f;
+        __syncthreads();
+    }
     for (int rel = 0; rel < steps; ++rel) {
         const int t = t0 + rel;
         if (blockIdx.x == 0 && tid == 0) stamp_step(d.ctl, rel);
         // det mode computes the step's thalamic draws while the accumulation's
         // first bitmap loads are in flight (18.25 -> 18.0 us per step at
-        // N=100k); in atomic mode that ran slower (13.35 -> 13.8), and the
-        // draws follow the accumulation.
+        // N=100k); in atomic mode the draws follow the accumulation, which
+        // measured faster than overlapping them with the bitmap loads (13.35
+        // -> 13.8) or with the segment-bound loads (9.23 -> 9.96).  Atomic
+        // mode clears each neuron's input as the update reads it.
         float I0[R];
         using Plan = SlicePlan<kDet, kLarge>;
         if constexpr (kDet) {
@@ -1636,7 +1671,8 @@ __global__ void __launch_bounds__(1024) k_skl_slice(DevNet d, int t0, int steps,
             };
             slice_accumulate<kDet, Plan::kEPT, Plan::kSPT, Plan::kOwner, true>(d, t, p, lo, len, dyn, draws, &backoff);
         } else {
-            slice_accumulate<kDet, Plan::kEPT, Plan::kSPT, Plan::kOwner, true>(d, t, p, lo, len, dyn, NoPre(), &backoff);
+            slice_accumulate<kDet, Plan::kEPT, Plan::kSPT, Plan::kOwner, true, true>(d, t, p, lo, len, dyn, NoPre(),
+                                                                                    &backoff);
         }
         uint64_t masks[R];
 #pragma unroll
@@ -1650,6 +1686,7 @@ __global__ void __launch_bounds__(1024) k_skl_slice(DevNet d, int t0, int steps,
                 if constexpr (!kDet) I0[k] = g[k] * thalamic(d.seed, (uint32_t)t, (uint32_t)j);
                 float I = I0[k];  // g * z, as neuron_step
                 I = I + acc[off];
+                if constexpr (!kDet) acc[off] = 0.0f;
                 spk = izh_step(v[k], u[k], a[k], b[k], c[k], dd[k], I);
                 nos[k] += spk ? 1 : 0;
                 if (t == last_t) d.lastI[j] = I;
