@@ -30,14 +30,14 @@ enum { kCtlCount = 0, kCtlStamp = 2 };
 // waits) followed by the control rows, so that one copy brings both back.
 constexpr int kCtlHead = 4;
 constexpr uint32_t kSpinLimit = 1u << 24;     // bounded device waits (~10 s)
-// Grid-barrier arrival counters: one per group of workgroups (blockIdx % 16,
-// two groups per XCD), each in its own 256-byte line.  Measured at N=100k,
-// 241 workgroups: 16 counters 13.18-13.20 us per SKL step, 8 counters
-// 13.43-13.47, 12 counters 13.23-13.30, 24 counters 13.82, 32 counters
-// 13.72 (det 17.7, 18.1, 17.9, 17.85, 17.7).
+// Grid barrier: one arrival counter and one release flag per group of
+// workgroups (blockIdx % 16, two groups per XCD), each in its own 256-byte
+// line; the counters first, then the flags.  Measured on MI355X, 241
+// workgroups of 1024 threads, barrier alone: 2.02-2.05 us with 16 groups,
+// 2.14 with 8, 2.09 with 32; counters polled directly (no flags) 2.67-2.99.
 constexpr int kBarGroups = 16;
 constexpr int kBarStride = 64;
-constexpr int kBarWords = kBarGroups * kBarStride;
+constexpr int kBarWords = 2 * kBarGroups * kBarStride;
 
 // Everything a kernel needs about one network, passed by value.
 //  - neuron state/parameters are structure-of-arrays fp32 (the reference's

@@ -184,22 +184,25 @@ __device__ __forceinline__ void push_flat(const DevNet &d, int t, int count, int
     }
 }
 
-// Grid-wide barrier for cooperative launches.  Workgroup b arrives on
-// counter b % kBarGroups (16 counters, two per XCD): device-scope atomics on
-// one address are serialised, so counters in separate lines each take a
-// share of the arrivals.  Counters only increase; at epoch e group g is
-// complete when its counter reaches e * (number of workgroups in g).  Lanes
-// 0..kBarGroups-1 of the first wave poll the counters together, and lane
-// kBarGroups the error flag in the same round trip.  Bounded: returns false
-// (and raises the error flag) if the wait exceeds kSpinLimit, or returns
-// false once another workgroup has raised the flag.
-// It orders only data written and read with agent-scope atomics (the spike
+// Grid-wide barrier of a persistent launch.  Workgroup b arrives on the
+// counter of its group b % kBarGroups (two groups per XCD) with a returning
+// atomic add; the group's last arriver of the epoch (old value e*m - 1, m
+// workgroups in the group) stores e into the group's release flag.  Lanes
+// 0..kBarGroups-1 of the first wave poll the flags together, and lane
+// kBarGroups the error flag in the same round trip, until every flag holds
+// the epoch.  The polled lines change once per barrier, not at every
+// arrival, which measured 0.6-0.9 us faster than polling the counters
+// themselves (241 workgroups).  Bounded: returns false (and raises the
+// error flag) if the wait exceeds kSpinLimit, or returns false once another
+// workgroup has raised the flag.
+// It orders only data written and read with agent-scope accesses (the spike
 // bitmap of the SKL kernel): every wave waits for its own stores to complete
-// before the workgroup arrives, and the loads after the barrier are issued
-// only once the final arrival has been seen.  It pays no release (an L2
-// write-back of the XCD) and no acquire (an L2 invalidate, which would also
-// evict the static synapse rows that the next step reads).  Between its two
-// workgroup barriers a non-polling wave stores the step's spike count.
+// before the workgroup arrives, a group's flag is stored only after all of
+// its members' arrivals, and the loads after the barrier are issued only
+// once every flag has been seen.  It pays no release (an L2 write-back of
+// the XCD) and no acquire (an L2 invalidate, which would also evict the
+// static synapse rows that the next step reads).  Between its two workgroup
+// barriers a non-polling wave stores the step's spike count.
 static_assert(kBarGroups < kWave, "the barrier's polling wave also reads the error flag");
 __device__ bool grid_sync(const DevNet &d, uint32_t &epoch, const int *s_wcnt, int32_t *cnt_dst)
 {
@@ -218,20 +221,26 @@ __device__ bool grid_sync(const DevNet &d, uint32_t &epoch, const int *s_wcnt, i
     if (threadIdx.x < kWave) {
         const int lane = (int)threadIdx.x;
         const uint32_t G = gridDim.x;
-        if (lane == 0)
-            __hip_atomic_fetch_add(&d.bar[(blockIdx.x % kBarGroups) * kBarStride], 1u, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t members = G / kBarGroups + ((uint32_t)lane < G % kBarGroups ? 1u : 0u);
-        const uint32_t target = epoch * members;
-        // lanes 0..kBarGroups-1 read the counters and lane kBarGroups the
-        // error flag, in one round trip per poll
-        const uint32_t *src = lane < kBarGroups ? &d.bar[lane * kBarStride]
-                                                : (lane == kBarGroups ? (const uint32_t *)d.err : &d.bar[0]);
+        uint32_t *flags = d.bar + kBarGroups * kBarStride;
+        if (lane == 0) {
+            const uint32_t grp = blockIdx.x % kBarGroups;
+            const uint32_t members = G / kBarGroups + (grp < G % kBarGroups ? 1u : 0u);
+            const uint32_t old = __hip_atomic_fetch_add(&d.bar[grp * kBarStride], 1u, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT);
+            if (old + 1u == epoch * members)
+                __hip_atomic_store(&flags[grp * kBarStride], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        // lanes 0..kBarGroups-1 read the flags and lane kBarGroups the
+        // error flag, in one round trip per poll; groups without members
+        // (grids smaller than kBarGroups) are never waited on
+        const bool waits = lane < kBarGroups && (uint32_t)lane < G;
+        const uint32_t *src = lane < kBarGroups ? &flags[lane * kBarStride]
+                                                : (lane == kBarGroups ? (const uint32_t *)d.err : &flags[0]);
         int ok = 1;
         uint32_t spins = 0;
         for (;;) {
             const uint32_t x = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (__all(lane >= kBarGroups || x >= target)) break;
+            if (__all(!waits || x >= epoch)) break;
             if (__any(lane == kBarGroups && x != 0u)) {  // another workgroup gave up
                 ok = 0;
                 break;
@@ -241,7 +250,7 @@ __device__ bool grid_sync(const DevNet &d, uint32_t &epoch, const int *s_wcnt, i
                 ok = 0;
                 break;
             }
-            __builtin_amdgcn_s_sleep(2);
+            __builtin_amdgcn_s_sleep(1);
         }
         if (lane == 0) s_ok = ok;
     }
