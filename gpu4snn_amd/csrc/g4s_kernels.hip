@@ -714,11 +714,22 @@ constexpr bool kOwnerSingle = false, kOwnerBatch = true;
 //           spikes per wave): a map u8 x kWaveMap per wave after the search
 //           lookup's offsets; the offsets region holds each wave's spike ids
 //           and the base region each wave's segment bases
+//   det, wave-local (the same plans): after the cursors, 16-byte aligned, a
+//           map u8 x kDetMap per wave, then keys u32 and weights f32 for
+//           kDetMap entries per wave, and 64 bytes that the sum's 16-wide
+//           reads may pass over
 constexpr int kWaveMap = 384;  // entries per wave-local chunk: 6 per lane
+constexpr int kDetSlots = 6;   // det, wave-local: entries per lane, all in registers
+constexpr int kDetMap = kDetSlots * 64;
 __host__ __device__ constexpr bool wave_local_plan(bool det, int spt, bool owner)
 {
     return !det && !owner && 64 * spt < 256;
 }
+__host__ __device__ constexpr bool det_local_plan(bool det, int spt, bool owner)
+{
+    return det && !owner && 64 * spt < 256;
+}
+__host__ __device__ inline size_t det_local_offset(size_t det_end) { return (det_end + 15) & ~(size_t)15; }
 __host__ __device__ inline int acc_floats(int L) { return (L + 3) & ~3; }
 __host__ __device__ inline size_t scan_bytes(int block) { return ((size_t)(block / 64 + 1) * 4 + 15) & ~(size_t)15; }
 __host__ __device__ inline size_t slice_lds_bytes(int L, int block, bool det, int ept, int spt, bool owner)
@@ -726,6 +737,8 @@ __host__ __device__ inline size_t slice_lds_bytes(int L, int block, bool det, in
     const size_t cap = (size_t)spt * (size_t)block, wd = (size_t)ept * (size_t)block;
     size_t b = (size_t)acc_floats(L) * 4 + cap * 8 + scan_bytes(block);
     if (!owner) b += (cap * 4 + 15) & ~(size_t)15;
+    if (det && det_local_plan(det, spt, owner))
+        return det_local_offset(b + wd * 6 + (size_t)L * 8) + (size_t)(block / 64) * kDetMap * 9 + 64;
     if (det) return b + wd * 6 + (size_t)L * 8;
     if (wave_local_plan(det, spt, owner)) b += (size_t)(block / 64) * kWaveMap;
     return b + (owner ? wd * 4 : 0);
@@ -954,6 +967,164 @@ __device__ __forceinline__ void wave_accumulate(const Net &d, int p, int lo, flo
     }
 }
 
+// Det mode, wave-local: one wave's spikes and entries, all in registers.
+// The same steps as wave_accumulate (ids, segment bounds, map) for a single
+// batch and a single chunk of at most kDetSlots*64 entries; entry
+// c0 + 64i + lane lands in x[i] (target 0xFFFFFFFF past the wave's last)
+// with the id of its source neuron in key[i].  Returns false, uniform over
+// the wave, if the wave holds more than 64*kSPT spikes or more entries than
+// that (nothing is loaded then).
+template <bool kCoh, int kSPT, class Net>
+__device__ __forceinline__ bool wave_collect(const Net &d, int p, uint4 kept0, uint4 kept1, int g0, uint32_t *wids,
+                                             uint64_t *wbs, uint8_t *wmap, uint2 (&x)[kDetSlots],
+                                             uint32_t (&key)[kDetSlots])
+{
+    constexpr int kWaveCap = kWave * kSPT;
+    static_assert(kWaveCap <= 256, "wave-local spike indices are bytes");
+    const int lane = lane_id();
+    const uint32_t stride = (uint32_t)d.P + 1u;
+#pragma unroll
+    for (int i = 0; i < kDetSlots; ++i) {
+        x[i] = make_uint2(0xFFFFFFFFu, 0u);
+        key[i] = 0u;
+    }
+    const int pk = popc4(kept0) | (popc4(kept1) << 16);
+    const int incl = wave_inclusive_sum(pk);
+    const int tot = wave_last(incl);
+    const int T0 = tot & 0xFFFF;
+    const int wtot = T0 + (tot >> 16);
+    if (wtot > kWaveCap) return false;  // uniform
+    if (wtot == 0) return true;
+    const int ex = incl - pk;
+    put_group<true>(kept0, g0, ex & 0xFFFF, wids, 0, 0);
+    put_group<true>(kept1, g0 + kWave, T0 + (ex >> 16), wids, 0, 0);
+    wave_sync();
+    uint32_t sid[kSPT], a0[kSPT], a1[kSPT];
+#pragma unroll
+    for (int q = 0; q < kSPT; ++q) sid[q] = lane + kWave * q < wtot ? wids[lane + kWave * q] : 0u;
+#pragma unroll
+    for (int q = 0; q < kSPT; ++q) {
+        const auto o = d.soff + ((uint64_t)sid[q] * stride + (uint32_t)p);
+        a0[q] = o[0];
+        a1[q] = o[1];
+    }
+    int len[kSPT], run[kSPT];
+    int wlen = 0;
+#pragma unroll
+    for (int q = 0; q < kSPT; ++q) {
+        len[q] = lane + kWave * q < wtot ? (int)(a1[q] - a0[q]) : 0;
+        const int inc = wave_inclusive_sum(len[q]);
+        run[q] = wlen + inc - len[q];
+        wlen += wave_last(inc);
+    }
+    if (wlen > kDetMap) return false;  // uniform
+#pragma unroll
+    for (int q = 0; q < kSPT; ++q)
+        if (len[q] > 0) {
+            wbs[lane + kWave * q] = (uint64_t)sid[q] * (uint64_t)d.nsyn + (uint64_t)a0[q] - (uint64_t)run[q];
+            for (int f = run[q]; f < run[q] + len[q]; ++f) wmap[f] = (uint8_t)(lane + kWave * q);
+        }
+    wave_sync();
+    uint32_t jm[kDetSlots];
+#pragma unroll
+    for (int i = 0; i < kDetSlots; ++i) jm[i] = kWave * i + lane < wlen ? (uint32_t)wmap[kWave * i + lane] : 0u;
+    uint64_t idx[kDetSlots];
+#pragma unroll
+    for (int i = 0; i < kDetSlots; ++i) {
+        idx[i] = wbs[jm[i]] + (uint64_t)(kWave * i + lane);
+        key[i] = wids[jm[i]];
+    }
+#pragma unroll
+    for (int i = 0; i < kDetSlots; ++i)
+        if (kWave * i + lane < wlen) x[i] = entry_at(d.syn, (size_t)idx[i]);
+    return true;
+}
+
+// Det mode: acc[j] <- the entries of x (targets j + lo, counted into dcur
+// before the workgroup barrier that precedes the call) added in ascending
+// key order from acc[j], for keys unique per target.  A counting sort by
+// target: bucket starts (a scan over the targets, this thread owning
+// [tb, te)), keys into the buckets, each entry's rank among its bucket's
+// keys puts its weight in key order, then every target adds its bucket in
+// order.  dkey and dw hold every entry of the call.  Leaves dcur zero; ends
+// with a workgroup barrier.
+template <int K>
+__device__ __forceinline__ void det_sort_add(float *acc, int lo, const uint2 (&x)[K], const uint32_t (&key)[K],
+                                             int *dcur, int *dstart, uint32_t *dkey, float *dw, int *scratch, int tb,
+                                             int te)
+{
+    int csum = 0;
+    for (int j = tb; j < te; ++j) csum += dcur[j];
+    int all = 0;
+    int st = block_exclusive_scan(csum, scratch, &all);
+    for (int j = tb; j < te; ++j) {
+        const int c = dcur[j];
+        dstart[j] = st;
+        dcur[j] = st;
+        st += c;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < K; ++i)
+        if (x[i].x != 0xFFFFFFFFu) dkey[atomicAdd(&dcur[(int)x[i].x - lo], 1)] = key[i];
+    __syncthreads();
+    // Each entry's rank among its bucket's keys puts its weight in key
+    // order.  The bounds of every slot's bucket are read first, then per
+    // slot 8 keys from the bucket's 16-byte aligned start (reads past a
+    // bucket's end stay inside the LDS workspace and are masked); longer
+    // buckets read the rest in a loop.
+    uint32_t se[K];  // bucket start | end << 16 (a step holds fewer than 65536 entries)
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+        const int j = x[i].x != 0xFFFFFFFFu ? (int)x[i].x - lo : 0;
+        se[i] = (uint32_t)dstart[j] | ((uint32_t)dcur[j] << 16);
+    }
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+        // one slot at a time: the scheduler may not hoist the next slot's
+        // key reads above this point, which keeps the slots' registers apart
+        __builtin_amdgcn_sched_barrier(0);
+        if (x[i].x == 0xFFFFFFFFu) continue;
+        const int s0 = (int)(se[i] & 0xFFFFu), s1 = (int)(se[i] >> 16);
+        const int q0 = s0 & ~3;
+        const uint4 ka = *(const uint4 *)(dkey + q0), kb = *(const uint4 *)(dkey + q0 + 4);
+        const uint32_t h[8] = {ka.x, ka.y, ka.z, ka.w, kb.x, kb.y, kb.z, kb.w};
+        int r = 0;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) r += ((uint32_t)(q0 + e - s0) < (uint32_t)(s1 - s0) && h[e] < key[i]) ? 1 : 0;
+        for (int q = q0 + 8; q < s1; q += 4) {
+            const uint4 kv = *(const uint4 *)(dkey + q);
+            const uint32_t g4[4] = {kv.x, kv.y, kv.z, kv.w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) r += (q + e < s1 && g4[e] < key[i]) ? 1 : 0;
+        }
+        dw[s0 + r] = u2f(x[i].y);
+    }
+    __syncthreads();
+    // each target adds its bucket in order, its weights read 16 at a time
+    for (int j = tb; j < te; ++j) {
+        const int b0 = dstart[j], b1 = dcur[j];
+        float a = acc[j];
+        for (int q0 = b0 & ~3; q0 < b1; q0 += 16) {
+            float4 wv[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) wv[c] = *(const float4 *)(dw + q0 + 4 * c);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const float h[4] = {wv[c].x, wv[c].y, wv[c].z, wv[c].w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int q = q0 + 4 * c + e;
+                    if (q >= b0 && q < b1) a = a + h[e];
+                }
+            }
+        }
+        acc[j] = a;
+        dcur[j] = 0;
+    }
+    __syncthreads();
+}
+
 // acc[0..len) <- input of slice p from the rows that spiked at step t-1.
 // Three dependent global levels, each with all its loads in flight:
 //  1. the step's bitmap, compacted in LDS into the ascending spike list
@@ -1044,6 +1215,9 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
     // the caller's work that needs no spikes (the neurons' thalamic draws)
     // runs while the bitmap loads are in flight
     pre();
+    // targets owned by this thread in the ordered phases: a contiguous run
+    const int tper = (len + B - 1) / B;
+    const int tb = min(tid * tper, len), te = min(tb + tper, len);
     constexpr bool kWaveLocal = wave_local_plan(kDet, kSPT, kOwner);
     if constexpr (kWaveLocal) {
         if (G <= 2) {  // uniform: every group of the wave is in registers
@@ -1055,15 +1229,41 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
             return;
         }
     }
+    constexpr bool kDetLocal = det_local_plan(kDet, kSPT, kOwner);
+    if constexpr (kDetLocal) {
+        // Every wave collects its own entries, with their source ids as
+        // keys, and one counting sort over the step orders each target's
+        // entries by source.  If a wave holds more than fits in registers,
+        // the windows below do the step instead.
+        if (G <= 2) {  // uniform
+            __shared__ int s_fit[1024 / kWave];
+            char *wl = (char *)dyn + det_local_offset((size_t)(w2 - (char *)dyn) + (size_t)Wd * 6 + (size_t)L * 8);
+            uint32_t *dkey32 = (uint32_t *)(wl + (size_t)nwaves * kDetMap);
+            float *dw32 = (float *)(wl + (size_t)nwaves * kDetMap * 5);
+            uint2 x[kDetSlots];
+            uint32_t key[kDetSlots];
+            const bool fits = wave_collect<kCoh, kSPT>(d, p, kept0, kept1, g0, (uint32_t *)segp + wave * kWave * kSPT,
+                                                       base + wave * kWave * kSPT,
+                                                       (uint8_t *)wl + (size_t)wave * kDetMap, x, key);
+            if (lane == 0) s_fit[wave] = fits ? 1 : 0;
+            __syncthreads();  // also orders the cursors' clearing before the counts
+            const bool all_fit = __ballot(lane < nwaves && s_fit[lane] == 0) == 0;  // uniform
+            if (all_fit) {
+#pragma unroll
+                for (int i = 0; i < kDetSlots; ++i)
+                    if (x[i].x != 0xFFFFFFFFu) atomicAdd(&dcur[(int)x[i].x - lo], 1);
+                __syncthreads();
+                det_sort_add<kDetSlots>(acc, lo, x, key, dcur, dstart, dkey32, dw32, scratch, tb, te);
+                return;
+            }
+        }
+    }
     int mine = popc4(kept0) + popc4(kept1);
     for (int k = 2; k < G; k += 2) {
         const uint4 xa = group(k), xb = group(k + 1);
         mine += popc4(xa) + popc4(xb);
     }
     const uint32_t stride = (uint32_t)d.P + 1u;
-    // targets owned by this thread in the ordered phases: a contiguous run
-    const int tper = (len + B - 1) / B;
-    const int tb = min(tid * tper, len), te = min(tb + tper, len);
     int total = 0;
     int wrun = 0, wsum = 0;  // general path: spikes before this wave, in it
     uint32_t sid[kSPT], e0[kSPT];
