@@ -349,16 +349,19 @@ int build_slices(g4s_net *net, int want)
     return G4S_OK;
 }
 
-// Workgroup size of the slice kernels: requested, or one neuron per thread
-// up to 1024 threads.
-// Slice kernels default to one thread per neuron of the slice (at most 1024).
-// More threads per slice run faster (the autotuner picks 1024 at N=1e5), but
-// in atomic mode they let the adds of a target arrive further from source
-// order, and the first steps then drift from the ordered sum sooner.
-int slice_block(const g4s_net *net, int requested)
+// Workgroup size of the slice kernels: requested, or by default 1024
+// threads in det mode and one thread per neuron of the slice (in whole
+// waves, at most 1024) in atomic mode.  More threads per slice run faster
+// (the autotuner picks 1024 at N=1e5; det at N=1e5 takes 13.1 us per step
+// with 1024 threads and 21.6 with one thread per neuron, whose waves then
+// hold more entries than the wave-local accumulation keeps in registers).
+// Det results do not depend on the thread count; in atomic mode more threads
+// let the adds of a target arrive further from source order, and the first
+// steps then drift from the ordered sum sooner.
+int slice_block(const g4s_net *net, int requested, bool det)
 {
     if (requested > 0) return requested;
-    // one thread per slice neuron, in whole waves
+    if (det) return 1024;
     return (int)std::min<int64_t>(1024, std::max<int64_t>(64, ((int64_t)net->d.L + 63) / 64 * 64));
 }
 
@@ -680,17 +683,17 @@ int run_impl(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, 
         // slice scheme); building one would allocate
         if (!net->slices_ready)
             return fail(G4S_ERR_STATE, "no slice partition: run g4s_run once with this scheme and options first");
-        sblock = slice_block(net, o.block);
+        sblock = slice_block(net, o.block, det);
     } else if (slice && o.slices > 0) {
         if ((rc = build_slices(net, o.slices))) return rc;
-        sblock = slice_block(net, o.block);
+        sblock = slice_block(net, o.block, det);
     } else if (slice) {
         // default tiling: about one slice per CU; more slices while a
         // slice's workspace would not fit the LDS of one workgroup
         if (net->cus == 0) net->cus = device_cus();
         for (int64_t want = std::max(net->cus, 1);; want *= 2) {
             if ((rc = build_slices(net, (int)want))) return rc;
-            sblock = slice_block(net, o.block);
+            sblock = slice_block(net, o.block, det);
             if (slice_lds(net->d.L, sblock, det) <= kSliceLdsMax || net->d.L <= 64) break;
         }
     }
