@@ -45,6 +45,19 @@ def test_table_covers_slice_kernels(g4s, name):
         assert r["waves_per_simd"] >= 4, (name, targs, r)
 
 
+def test_one_neuron_per_thread_slice_kernels_do_not_spill(g4s):
+    """The slice kernels that the default single-network plans run (one
+    neuron per thread: slices up to the workgroup size) and the batched
+    kernels keep every value in registers: no scratch memory.  A build whose
+    per-step det kernel spilled gave wrong det results."""
+    names = [(n, f"1,{det},false") for n in ("k_skl_slice", "k_slice_step") for det in ("false", "true")]
+    names += [("k_batch_step", f"{det},{b}") for det in ("false", "true") for b in (256, 512, 1024)]
+    names += [("k_batch_chunk", det) for det in ("false", "true")]
+    for name, targs in names:
+        r = g4s.kernel_resources(name, targs)
+        assert r["scratch"] == 0, (name, targs, r)
+
+
 def test_table_covers_persistent_ap(g4s):
     r = g4s.kernel_resources("k_ap", "")
     assert 1 <= r["waves_per_simd"] <= 8
