@@ -189,10 +189,13 @@ def bench_batch(g, args):
         "speedup_vs_plain": (sim_s / wall) / (p_sim / p_wall),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
+                     # profiled at the untuned plan (768-neuron slices); the mix
+                     # tuner moves one or two of 55 buckets of this mix
                      "traffic": (traffic_record(f"k_batch_step|batch{len(specs)}|{args.accum}|chunk={args.chunk}"
                                                 f"|slice_len=768|streams={args.streams or 3}|per_step") or {})
                      .get("bytes_per_step")
-                     if not args.slice_len and not args.tune and args.schedule == "step" and args.seed == 20221 else None,
+                     if not args.slice_len and args.schedule == "step" and args.seed == 20221 else None,
+                     "traffic_plan": "768-neuron slices, untuned",
                      "kernel": "k_batch_step",
                      # each step is one launch per instance group, the groups'
                      # launches running concurrently on their own streams:
