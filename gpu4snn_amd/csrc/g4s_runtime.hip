@@ -151,6 +151,10 @@ struct g4s_net {
     hipEvent_t ev0[2] = {nullptr, nullptr}, ev1[2] = {nullptr, nullptr}, done_ev[2] = {nullptr, nullptr};
     int64_t t = 0;
     int chunk_cap = 0;             // raster rows / ctl entries allocated
+    // The control rows (chunk_cap + 1 of them) and the barrier words are
+    // zero: cleared after the last chunk's results were queued for copy, so
+    // that the next chunk starts without a clearing launch.
+    bool ctl_clean = false;
     uint32_t *h_raster[2] = {nullptr, nullptr};
     int32_t *h_ctl[2] = {nullptr, nullptr};  // control block copies: error flag, then the rows
     bool slices_ready = false;
@@ -253,6 +257,7 @@ int ensure_chunk(g4s_net *net, int chunk)
         net->h_ctl[k] = nullptr;
     }
     net->d.raster = nullptr; net->d.ctl = nullptr; net->d.err = nullptr;
+    net->ctl_clean = false;
     int rc;
     if ((rc = net->alloc(&net->d.raster, (size_t)chunk * net->d.words))) return rc;
     int32_t *blk = nullptr;
@@ -835,7 +840,12 @@ int run_impl(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, 
         const int t0 = (int)(net->t);
         const int buf = chunk_no & 1;
         if (slice_steps && use_graph && (rc = prepare_ab_graph(net, cs, sblock, det))) return abandon(rc);
-        HIP_OK(launch_fill2_u32((uint32_t *)d.ctl, (size_t)(cs + 1) * kCtlStride, d.bar, kBarWords, 0u, net->stream));
+        // control rows and barrier words start at zero: cleared here only if
+        // the previous chunk did not leave them so (and always for the
+        // caller-independent scratch of g4s_run_noalloc)
+        if (na || !net->ctl_clean)
+            HIP_OK(launch_fill2_u32((uint32_t *)d.ctl, (size_t)(cs + 1) * kCtlStride, d.bar, kBarWords, 0u, net->stream));
+        if (!na) net->ctl_clean = false;
         if (na) d.raster = na->raster ? na->raster + (size_t)c0 * (size_t)words : nullptr;
         if (slice_steps) {
             int32_t *ti = net->h_tinfo + 2 * buf;
@@ -876,6 +886,13 @@ int run_impl(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, 
         if (raster || fn)
             HIP_OK(hipMemcpyAsync(net->h_raster[buf], d.raster, (size_t)cs * words * sizeof(uint32_t), hipMemcpyDeviceToHost, net->stream));
         HIP_OK(hipEventRecord(net->done_ev[buf], net->stream));
+        if (!na) {
+            // clear for the next chunk, after this one's copies in stream
+            // order: the host's wait for done_ev does not include it
+            HIP_OK(launch_fill2_u32((uint32_t *)d.ctl, (size_t)(net->chunk_cap + 1) * kCtlStride, d.bar, kBarWords, 0u,
+                                    net->stream));
+            net->ctl_clean = true;
+        }
         net->t += cs;
         pend[npend++] = Pending{c0, cs, t0, buf};
         if (!overlap || npend == 2) {
