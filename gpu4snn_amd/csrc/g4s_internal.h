@@ -70,7 +70,7 @@ struct DevNet {
     const uint16_t *soff;
     int32_t P, L;
     // Per-slice spike counts of each step of the chunk ([chunk][P]), written
-    // without atomics and summed into ctl by k_reduce_counts.
+    // without atomics and summed into ctl by k_finish_chunk.
     int32_t *bcnt;
     // Per-step kernels replayed from a HIP graph read the chunk's first
     // step and the run's last step here: tinfo[0] = t0, tinfo[1] = last.
@@ -98,18 +98,20 @@ hipError_t launch_init(const DevNet &d, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, size_t n, uint32_t val, hipStream_t s);
 // Two ranges filled by one launch.
 hipError_t launch_fill2_u32(uint32_t *p, size_t n, uint32_t *q, size_t m, uint32_t val, hipStream_t s);
-// counts[r] = spike count of control row r, r < steps.
-hipError_t launch_copy_counts(const int32_t *ctl, int32_t *counts, int steps, hipStream_t s);
-// Stamp the end of a chunk of `rel` steps into control row `rel`.
-hipError_t launch_stamp(int32_t *ctl, int rel, hipStream_t s);
 hipError_t launch_update(const DevNet &d, int t, int rel, int last, const Launch &L);
 hipError_t launch_push_rows(const DevNet &d, int t, int rel, const Launch &L);
 // Scatter every row set in step t's bitmap into isyn[(t+1)&1] (rebuild of
 // the pending input buffer when a global-atomic scheme follows one that
 // keeps none).
 hipError_t launch_push_bitmap(const DevNet &d, int t, hipStream_t s);
-// Also stamps the chunk's end into control row `steps`.
-hipError_t launch_reduce_counts(const DevNet &d, int steps, hipStream_t s);
+// The end of a chunk: its end stamp into control row `steps`; with `reduce`,
+// each step's count summed from the per-slice counts (bcnt); then the
+// control block (the kCtlHead words at d.err, rows 0..steps) written to
+// `host` (device address of mapped pinned host memory) and, if `counts` is
+// not null, each step's count to counts[r].  `done`, if not null, is an
+// event that the dispatch's own end signals.
+hipError_t launch_finish_chunk(const DevNet &d, int steps, bool reduce, int32_t *host, int32_t *counts, hipEvent_t done,
+                               hipStream_t s);
 hipError_t launch_push_flat(const DevNet &d, int t, int rel, const Launch &L);
 hipError_t launch_ap(const DevNet &d, int t, int rel, int last, int prev_count, const Launch &L);
 hipError_t launch_slice_offsets(const DevNet &d, uint16_t *soff, hipStream_t s);

@@ -302,19 +302,6 @@ __global__ void __launch_bounds__(256) k_fill2_u32(uint32_t *p, size_t n, uint32
     }
 }
 
-// counts[r] = spikes of chunk step r, r < steps (from the control rows).
-__global__ void __launch_bounds__(256) k_copy_counts(const int32_t *ctl, int32_t *counts, int steps)
-{
-    for (int r = (int)(blockIdx.x * blockDim.x + threadIdx.x); r < steps; r += (int)(gridDim.x * blockDim.x))
-        counts[r] = ctl[r * kCtlStride + kCtlCount];
-}
-
-// The chunk's end stamp (control row `rel` = the chunk's length).
-__global__ void __launch_bounds__(64) k_stamp(int32_t *ctl, int rel)
-{
-    if (threadIdx.x == 0) stamp_step(ctl, rel);
-}
-
 // (a)+(b) for the atomic-mode per-step schemes N, S, AB.
 __global__ void __launch_bounds__(256) k_update(DevNet d, int t, int rel, int last)
 {
@@ -1572,26 +1559,54 @@ __global__ void __launch_bounds__(256) k_slice_offsets(DevNet d, uint16_t *soff)
     }
 }
 
-// ctl[rel].count = sum over slices of bcnt[rel][p]; one workgroup per step.
-// It runs right after the chunk's step kernels: its start is the chunk's end
-// stamp (control row `steps`).
-__global__ void __launch_bounds__(256) k_reduce_counts(DevNet d, int steps)
+// The end of a chunk of `steps` steps, right after its step kernels (its
+// start is the chunk's end stamp, control row `steps`).  With `reduce`
+// (slice schemes: per-slice counts in bcnt) ctl[rel].count = sum over slices
+// of bcnt[rel][p], one workgroup per step.  Then the control block -- the
+// kCtlHead words at d.err, then rows 0..steps -- is written to `host`, the
+// device address of mapped pinned host memory that the host reads after the
+// chunk's completion event (no copy follows), and, if `counts` is set, each
+// step's count to counts[rel] (a caller's device buffer).
+__device__ __forceinline__ void host_store(int32_t *p, int32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+__global__ void __launch_bounds__(256) k_finish_chunk(DevNet d, int steps, int reduce, int32_t *host, int32_t *counts)
 {
     __shared__ int part[256];
-    if (blockIdx.x == 0 && threadIdx.x == 0) stamp_step(d.ctl, steps);
-    for (int rel = blockIdx.x; rel < steps; rel += gridDim.x) {
-        int s = 0;
-        const int32_t *row = d.bcnt + (size_t)rel * (size_t)d.P;
-        for (int p = (int)threadIdx.x; p < d.P; p += blockDim.x) s += row[p];
-        part[threadIdx.x] = s;
+    const int tid = (int)threadIdx.x;
+    if (blockIdx.x == 0) {
+        if (tid == 0) stamp_step(d.ctl, steps);
         __syncthreads();
-        for (int w = (int)blockDim.x / 2; w > 0; w >>= 1) {
-            if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        // the head words and the end row, one per thread
+        if (tid < kCtlHead + kCtlStride) {
+            const int32_t v = tid < kCtlHead ? d.err[tid] : d.ctl[steps * kCtlStride + tid - kCtlHead];
+            host_store(&host[tid < kCtlHead ? tid : kCtlHead + steps * kCtlStride + tid - kCtlHead], v);
+        }
+    }
+    for (int rel = blockIdx.x; rel < steps; rel += gridDim.x) {
+        int32_t *row = d.ctl + (size_t)rel * kCtlStride;
+        if (reduce) {
+            int s = 0;
+            const int32_t *b = d.bcnt + (size_t)rel * (size_t)d.P;
+            for (int p = tid; p < d.P; p += blockDim.x) s += b[p];
+            part[tid] = s;
+            __syncthreads();
+            for (int w = (int)blockDim.x / 2; w > 0; w >>= 1) {
+                if (tid < w) part[tid] += part[tid + w];
+                __syncthreads();
+            }
+            if (tid == 0) row[kCtlCount] = part[0];
             __syncthreads();
         }
-        if (threadIdx.x == 0) d.ctl[rel * kCtlStride + kCtlCount] = part[0];
-        __syncthreads();
+        if (tid < kCtlStride) {
+            const int32_t v = row[tid];
+            host_store(&host[kCtlHead + rel * kCtlStride + tid], v);
+            if (tid == kCtlCount && counts) counts[rel] = v;
+        }
     }
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's stores have completed
 }
 
 // AB scheme: one launch per step (rel-th of the chunk; t = tinfo[0] + rel),
@@ -2164,12 +2179,6 @@ hipError_t launch_init(const DevNet &d, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_copy_counts(const int32_t *ctl, int32_t *counts, int steps, hipStream_t s)
-{
-    if (steps <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_copy_counts, dim3(grid_for(steps, 256)), dim3(256), 0, s, ctl, counts, steps);
-    return hipGetLastError();
-}
 
 hipError_t launch_fill2_u32(uint32_t *p, size_t n, uint32_t *q, size_t m, uint32_t val, hipStream_t s)
 {
@@ -2178,11 +2187,6 @@ hipError_t launch_fill2_u32(uint32_t *p, size_t n, uint32_t *q, size_t m, uint32
     return hipGetLastError();
 }
 
-hipError_t launch_stamp(int32_t *ctl, int rel, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, s, ctl, rel);
-    return hipGetLastError();
-}
 
 hipError_t launch_fill_u32(uint32_t *p, size_t n, uint32_t val, hipStream_t s)
 {
@@ -2208,10 +2212,16 @@ hipError_t launch_push_bitmap(const DevNet &d, int t, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_reduce_counts(const DevNet &d, int steps, hipStream_t s)
+hipError_t launch_finish_chunk(const DevNet &d, int steps, bool reduce, int32_t *host, int32_t *counts, hipEvent_t done,
+                               hipStream_t s)
 {
-    // one workgroup per step
-    hipLaunchKernelGGL(k_reduce_counts, dim3(steps < 4096 ? (steps > 0 ? steps : 1) : 4096), dim3(256), 0, s, d, steps);
+    // one workgroup per step; `done` (if set) is signalled by this dispatch's end
+    const dim3 grid(steps < 4096 ? (steps > 0 ? steps : 1) : 4096);
+    if (done)
+        hipExtLaunchKernelGGL(k_finish_chunk, grid, dim3(256), 0, s, nullptr, done, 0, d, steps, reduce ? 1 : 0, host,
+                              counts);
+    else
+        hipLaunchKernelGGL(k_finish_chunk, grid, dim3(256), 0, s, d, steps, reduce ? 1 : 0, host, counts);
     return hipGetLastError();
 }
 

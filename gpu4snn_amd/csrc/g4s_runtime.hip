@@ -157,6 +157,7 @@ struct g4s_net {
     bool ctl_clean = false;
     uint32_t *h_raster[2] = {nullptr, nullptr};
     int32_t *h_ctl[2] = {nullptr, nullptr};  // control block copies: error flag, then the rows
+    int32_t *h_ctl_dev[2] = {nullptr, nullptr};  // their device addresses (mapped pinned memory)
     bool slices_ready = false;
     // Per-step AB launches captured once per chunk length and device layout.
     hipGraph_t graph = nullptr;
@@ -255,6 +256,7 @@ int ensure_chunk(g4s_net *net, int chunk)
         if (net->h_ctl[k]) (void)hipHostFree(net->h_ctl[k]);
         net->h_raster[k] = nullptr;
         net->h_ctl[k] = nullptr;
+        net->h_ctl_dev[k] = nullptr;
     }
     net->d.raster = nullptr; net->d.ctl = nullptr; net->d.err = nullptr;
     net->ctl_clean = false;
@@ -274,6 +276,7 @@ int ensure_chunk(g4s_net *net, int chunk)
     for (int k = 0; k < 2; ++k) {
         HIP_OK(hipHostMalloc((void **)&net->h_raster[k], (size_t)chunk * net->d.words * sizeof(uint32_t)));
         HIP_OK(hipHostMalloc((void **)&net->h_ctl[k], ((size_t)kCtlHead + (size_t)(chunk + 1) * kCtlStride) * sizeof(int32_t)));
+        HIP_OK(hipHostGetDevicePointer((void **)&net->h_ctl_dev[k], net->h_ctl[k], 0));
     }
     net->chunk_cap = chunk;
     net->gen += 1;
@@ -875,17 +878,17 @@ int run_impl(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, 
             }
         }
         HIP_OK(hipEventRecord(net->ev1[buf], net->stream));
-        if (slice)
-            HIP_OK(launch_reduce_counts(d, cs, net->stream));  // stamps the chunk's end too
-        else
-            HIP_OK(launch_stamp(d.ctl, cs, net->stream));
-        if (na && na->counts) HIP_OK(launch_copy_counts(d.ctl, na->counts + c0, cs, net->stream));
-        // one copy: the error flag, then the chunk's control rows
-        HIP_OK(hipMemcpyAsync(net->h_ctl[buf], d.err, ((size_t)kCtlHead + (size_t)(cs + 1) * kCtlStride) * sizeof(int32_t),
-                              hipMemcpyDeviceToHost, net->stream));
-        if (raster || fn)
+        // one kernel: the chunk's end stamp, the step counts (summed over the
+        // slices for slice schemes), the error flag and control rows written
+        // straight to the host buffer, and the caller's counts.  Without a
+        // raster copy after it, its dispatch signals the chunk's done event.
+        const bool copy_raster = raster || fn;
+        HIP_OK(launch_finish_chunk(d, cs, slice, net->h_ctl_dev[buf], na && na->counts ? na->counts + c0 : nullptr,
+                                   copy_raster ? nullptr : net->done_ev[buf], net->stream));
+        if (copy_raster) {
             HIP_OK(hipMemcpyAsync(net->h_raster[buf], d.raster, (size_t)cs * words * sizeof(uint32_t), hipMemcpyDeviceToHost, net->stream));
-        HIP_OK(hipEventRecord(net->done_ev[buf], net->stream));
+            HIP_OK(hipEventRecord(net->done_ev[buf], net->stream));
+        }
         if (!na) {
             // clear for the next chunk, after this one's copies in stream
             // order: the host's wait for done_ev does not include it
