@@ -701,8 +701,9 @@ constexpr bool kOwnerSingle = false, kOwnerBatch = true;
 //           spikes per wave): a map u8 x kWaveMap per wave after the search
 //           lookup's offsets; the offsets region holds each wave's spike ids
 //           and the base region each wave's segment bases
-//   det, wave-local (the same plans): after the cursors, 16-byte aligned, a
-//           map u8 x kDetMap per wave, then keys u32 and weights f32 for
+//   det, wave-local (the same plans, slices below kDetLocalSlice): after
+//           the cursors, 16-byte aligned, a map u8 x kDetMap per wave, then
+//           a byte and a u16 count per (wave, target), then weights f32 for
 //           kDetMap entries per wave, and 64 bytes that the sum's 16-wide
 //           reads may pass over
 constexpr int kWaveMap = 384;  // entries per wave-local chunk: 6 per lane
@@ -717,6 +718,14 @@ __host__ __device__ constexpr bool det_local_plan(bool det, int spt, bool owner)
     return det && !owner && 64 * spt < 256;
 }
 __host__ __device__ inline size_t det_local_offset(size_t det_end) { return (det_end + 15) & ~(size_t)15; }
+// Det, wave-local: slices shorter than this (the per-(wave, target) bytes
+// and counts grow with the slice; longer slices take the windows).
+constexpr int kDetLocalSlice = 1024;
+__host__ __device__ inline size_t det_counts_offset(int L, int nw) { return ((size_t)nw * L + 15) & ~(size_t)15; }
+__host__ __device__ inline size_t det_weights_offset(int L, int nw)
+{
+    return (size_t)nw * kDetMap + ((det_counts_offset(L, nw) + (size_t)nw * L * 2 + 15) & ~(size_t)15);
+}
 __host__ __device__ inline int acc_floats(int L) { return (L + 3) & ~3; }
 __host__ __device__ inline size_t scan_bytes(int block) { return ((size_t)(block / 64 + 1) * 4 + 15) & ~(size_t)15; }
 __host__ __device__ inline size_t slice_lds_bytes(int L, int block, bool det, int ept, int spt, bool owner)
@@ -724,8 +733,9 @@ __host__ __device__ inline size_t slice_lds_bytes(int L, int block, bool det, in
     const size_t cap = (size_t)spt * (size_t)block, wd = (size_t)ept * (size_t)block;
     size_t b = (size_t)acc_floats(L) * 4 + cap * 8 + scan_bytes(block);
     if (!owner) b += (cap * 4 + 15) & ~(size_t)15;
-    if (det && det_local_plan(det, spt, owner))
-        return det_local_offset(b + wd * 6 + (size_t)L * 8) + (size_t)(block / 64) * kDetMap * 9 + 64;
+    if (det && det_local_plan(det, spt, owner) && L < kDetLocalSlice)
+        return det_local_offset(b + wd * 6 + (size_t)L * 8) + det_weights_offset(L, block / 64) +
+               (size_t)(block / 64) * kDetMap * 4 + 64;
     if (det) return b + wd * 6 + (size_t)L * 8;
     if (wave_local_plan(det, spt, owner)) b += (size_t)(block / 64) * kWaveMap;
     return b + (owner ? wd * 4 : 0);
@@ -957,31 +967,27 @@ __device__ __forceinline__ void wave_accumulate(const Net &d, int p, int lo, flo
 // Det mode, wave-local: one wave's spikes and entries, all in registers.
 // The same steps as wave_accumulate (ids, segment bounds, map) for a single
 // batch and a single chunk of at most kDetSlots*64 entries; entry
-// c0 + 64i + lane lands in x[i] (target 0xFFFFFFFF past the wave's last)
-// with the id of its source neuron in key[i].  Returns false, uniform over
-// the wave, if the wave holds more than 64*kSPT spikes or more entries than
-// that (nothing is loaded then).
+// c0 + 64i + lane lands in x[i] (target 0xFFFFFFFF past the wave's last).
+// Returns, uniform over the wave: 0 if the wave holds more than 64*kSPT
+// spikes or more entries than that (nothing is loaded then), 1 if it holds
+// no entry, 2 if its entries' loads are in flight.
 template <bool kCoh, int kSPT, class Net>
-__device__ __forceinline__ bool wave_collect(const Net &d, int p, uint4 kept0, uint4 kept1, int g0, uint32_t *wids,
-                                             uint64_t *wbs, uint8_t *wmap, uint2 (&x)[kDetSlots],
-                                             uint32_t (&key)[kDetSlots])
+__device__ __forceinline__ int wave_collect(const Net &d, int p, uint4 kept0, uint4 kept1, int g0, uint32_t *wids,
+                                             uint64_t *wbs, uint8_t *wmap, uint2 (&x)[kDetSlots])
 {
     constexpr int kWaveCap = kWave * kSPT;
     static_assert(kWaveCap <= 256, "wave-local spike indices are bytes");
     const int lane = lane_id();
     const uint32_t stride = (uint32_t)d.P + 1u;
 #pragma unroll
-    for (int i = 0; i < kDetSlots; ++i) {
-        x[i] = make_uint2(0xFFFFFFFFu, 0u);
-        key[i] = 0u;
-    }
+    for (int i = 0; i < kDetSlots; ++i) x[i] = make_uint2(0xFFFFFFFFu, 0u);
     const int pk = popc4(kept0) | (popc4(kept1) << 16);
     const int incl = wave_inclusive_sum(pk);
     const int tot = wave_last(incl);
     const int T0 = tot & 0xFFFF;
     const int wtot = T0 + (tot >> 16);
-    if (wtot > kWaveCap) return false;  // uniform
-    if (wtot == 0) return true;
+    if (wtot > kWaveCap) return 0;  // uniform
+    if (wtot == 0) return 1;
     const int ex = incl - pk;
     put_group<true>(kept0, g0, ex & 0xFFFF, wids, 0, 0);
     put_group<true>(kept1, g0 + kWave, T0 + (ex >> 16), wids, 0, 0);
@@ -1004,7 +1010,8 @@ __device__ __forceinline__ bool wave_collect(const Net &d, int p, uint4 kept0, u
         run[q] = wlen + inc - len[q];
         wlen += wave_last(inc);
     }
-    if (wlen > kDetMap) return false;  // uniform
+    if (wlen > kDetMap) return 0;  // uniform
+    if (wlen == 0) return 1;
 #pragma unroll
     for (int q = 0; q < kSPT; ++q)
         if (len[q] > 0) {
@@ -1017,80 +1024,21 @@ __device__ __forceinline__ bool wave_collect(const Net &d, int p, uint4 kept0, u
     for (int i = 0; i < kDetSlots; ++i) jm[i] = kWave * i + lane < wlen ? (uint32_t)wmap[kWave * i + lane] : 0u;
     uint64_t idx[kDetSlots];
 #pragma unroll
-    for (int i = 0; i < kDetSlots; ++i) {
-        idx[i] = wbs[jm[i]] + (uint64_t)(kWave * i + lane);
-        key[i] = wids[jm[i]];
-    }
+    for (int i = 0; i < kDetSlots; ++i) idx[i] = wbs[jm[i]] + (uint64_t)(kWave * i + lane);
 #pragma unroll
     for (int i = 0; i < kDetSlots; ++i)
         if (kWave * i + lane < wlen) x[i] = entry_at(d.syn, (size_t)idx[i]);
-    return true;
+    return 2;
 }
 
-// Det mode: acc[j] <- the entries of x (targets j + lo, counted into dcur
-// before the workgroup barrier that precedes the call) added in ascending
-// key order from acc[j], for keys unique per target.  A counting sort by
-// target: bucket starts (a scan over the targets, this thread owning
-// [tb, te)), keys into the buckets, each entry's rank among its bucket's
-// keys puts its weight in key order, then every target adds its bucket in
-// order.  dkey and dw hold every entry of the call.  Leaves dcur zero; ends
-// with a workgroup barrier.
-template <int K>
-__device__ __forceinline__ void det_sort_add(float *acc, int lo, const uint2 (&x)[K], const uint32_t (&key)[K],
-                                             int *dcur, int *dstart, uint32_t *dkey, float *dw, int *scratch, int tb,
-                                             int te)
+// Det mode, last phase: each target j in [tb, te) adds its bucket
+// dw[dstart[j] .. dend[j]) in order, the weights read 16 at a time (reads
+// may pass the bucket's ends, within the workspace); dend is left zero.
+// Ends with a workgroup barrier.
+__device__ __forceinline__ void bucket_sums(float *acc, const float *dw, const int *dstart, int *dend, int tb, int te)
 {
-    int csum = 0;
-    for (int j = tb; j < te; ++j) csum += dcur[j];
-    int all = 0;
-    int st = block_exclusive_scan(csum, scratch, &all);
     for (int j = tb; j < te; ++j) {
-        const int c = dcur[j];
-        dstart[j] = st;
-        dcur[j] = st;
-        st += c;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < K; ++i)
-        if (x[i].x != 0xFFFFFFFFu) dkey[atomicAdd(&dcur[(int)x[i].x - lo], 1)] = key[i];
-    __syncthreads();
-    // Each entry's rank among its bucket's keys puts its weight in key
-    // order.  The bounds of every slot's bucket are read first, then per
-    // slot 8 keys from the bucket's 16-byte aligned start (reads past a
-    // bucket's end stay inside the LDS workspace and are masked); longer
-    // buckets read the rest in a loop.
-    uint32_t se[K];  // bucket start | end << 16 (a step holds fewer than 65536 entries)
-#pragma unroll
-    for (int i = 0; i < K; ++i) {
-        const int j = x[i].x != 0xFFFFFFFFu ? (int)x[i].x - lo : 0;
-        se[i] = (uint32_t)dstart[j] | ((uint32_t)dcur[j] << 16);
-    }
-#pragma unroll
-    for (int i = 0; i < K; ++i) {
-        // one slot at a time: the scheduler may not hoist the next slot's
-        // key reads above this point, which keeps the slots' registers apart
-        __builtin_amdgcn_sched_barrier(0);
-        if (x[i].x == 0xFFFFFFFFu) continue;
-        const int s0 = (int)(se[i] & 0xFFFFu), s1 = (int)(se[i] >> 16);
-        const int q0 = s0 & ~3;
-        const uint4 ka = *(const uint4 *)(dkey + q0), kb = *(const uint4 *)(dkey + q0 + 4);
-        const uint32_t h[8] = {ka.x, ka.y, ka.z, ka.w, kb.x, kb.y, kb.z, kb.w};
-        int r = 0;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) r += ((uint32_t)(q0 + e - s0) < (uint32_t)(s1 - s0) && h[e] < key[i]) ? 1 : 0;
-        for (int q = q0 + 8; q < s1; q += 4) {
-            const uint4 kv = *(const uint4 *)(dkey + q);
-            const uint32_t g4[4] = {kv.x, kv.y, kv.z, kv.w};
-#pragma unroll
-            for (int e = 0; e < 4; ++e) r += (q + e < s1 && g4[e] < key[i]) ? 1 : 0;
-        }
-        dw[s0 + r] = u2f(x[i].y);
-    }
-    __syncthreads();
-    // each target adds its bucket in order, its weights read 16 at a time
-    for (int j = tb; j < te; ++j) {
-        const int b0 = dstart[j], b1 = dcur[j];
+        const int b0 = dstart[j], b1 = dend[j];
         float a = acc[j];
         for (int q0 = b0 & ~3; q0 < b1; q0 += 16) {
             float4 wv[4];
@@ -1107,9 +1055,94 @@ __device__ __forceinline__ void det_sort_add(float *acc, int lo, const uint2 (&x
             }
         }
         acc[j] = a;
-        dcur[j] = 0;
+        dend[j] = 0;
     }
     __syncthreads();
+}
+
+// Det mode, stable placement: acc[j] <- acc[j] plus the entries of every
+// wave (x[i] is entry 64i + lane of this wave's concatenation, target j +
+// lo) added in the order (wave, i, lane), which is ascending source order:
+// each wave's spikes are ascending and precede the next wave's, and a
+// wave's entries follow its spikes.  No sort by key: an entry's place in
+// its target's bucket is the number of that target's entries in earlier
+// waves plus those before it in this wave (earlier slots, then lower lanes
+// of its own slot).  Per slot, the lanes that share a target find each other
+// through a wave-private byte per target (each stores its lane id, all read
+// back the same one of them) and ballots; a wave-private count per target
+// carries the place across slots.  scr (u8) and cnt (u16, zero on entry)
+// hold nwaves x L, wave-major; dstart and dend get the buckets' bounds.
+// Ends with a workgroup barrier.
+template <int K>
+__device__ __forceinline__ void det_stable_add(float *acc, int lo, int L, const uint2 (&x)[K], uint8_t *scr,
+                                               uint16_t *cnt, int *dstart, int *dend, float *dw, int *scratch,
+                                               int tb, int te)
+{
+    const int lane = lane_id(), wave = wave_index(), nw = (int)blockDim.x / kWave;
+    uint8_t *sw = scr + wave * L;
+    uint16_t *cw = cnt + wave * L;
+    int rk[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+        __builtin_amdgcn_sched_barrier(0);  // one slot at a time
+        const bool valid = x[i].x != 0xFFFFFFFFu;
+        const int j = valid ? (int)x[i].x - lo : 0;
+        if (valid) sw[j] = (uint8_t)lane;
+        // the byte read back may be another lane's store (the wave-level
+        // sync also keeps the compiler from reusing this lane's own value)
+        wave_sync();
+        const int rep = valid ? (int)sw[j] : -1;
+        const int prev = valid ? (int)cw[j] : 0;
+        uint64_t mine = valid ? (1ull << lane) : 0ull;
+        uint64_t pend = __ballot(valid && rep != lane);  // lanes whose target another lane shares
+        while (pend) {  // uniform: one pass per shared target
+            const int r = __builtin_amdgcn_readlane(rep, (int)__builtin_ctzll(pend));
+            const uint64_t grp = __ballot(valid && rep == r);
+            if (valid && rep == r) mine = grp;
+            pend &= ~grp;
+        }
+        const int below = bits_below(mine), n = __popcll(mine);
+        rk[i] = prev + below;
+        if (valid && below == n - 1) cw[j] = (uint16_t)(prev + n);
+    }
+    __syncthreads();
+    // buckets: all waves' counts, a scan over the targets, then each wave's
+    // first place in the bucket (in place of its count); the counts of a
+    // target are read four waves at a time
+    int csum = 0;
+    for (int j = tb; j < te; ++j) {
+        const uint16_t *cj = cnt + j;
+        for (int w0 = 0; w0 < nw; w0 += 4) {
+            int c[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) c[k] = w0 + k < nw ? (int)cj[(w0 + k) * L] : 0;
+            csum += (c[0] + c[1]) + (c[2] + c[3]);
+        }
+    }
+    int all = 0;
+    int st = block_exclusive_scan(csum, scratch, &all);
+    for (int j = tb; j < te; ++j) {
+        uint16_t *cj = cnt + j;
+        dstart[j] = st;
+        for (int w0 = 0; w0 < nw; w0 += 4) {
+            int c[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) c[k] = w0 + k < nw ? (int)cj[(w0 + k) * L] : 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (w0 + k < nw) {
+                    cj[(w0 + k) * L] = (uint16_t)st;
+                    st += c[k];
+                }
+        }
+        dend[j] = st;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < K; ++i)
+        if (x[i].x != 0xFFFFFFFFu) dw[(int)cw[(int)x[i].x - lo] + rk[i]] = u2f(x[i].y);
+    __syncthreads();
+    bucket_sums(acc, dw, dstart, dend, tb, te);
 }
 
 // acc[0..len) <- input of slice p from the rows that spiked at step t-1.
@@ -1218,29 +1251,29 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
     }
     constexpr bool kDetLocal = det_local_plan(kDet, kSPT, kOwner);
     if constexpr (kDetLocal) {
-        // Every wave collects its own entries, with their source ids as
-        // keys, and one counting sort over the step orders each target's
-        // entries by source.  If a wave holds more than fits in registers,
+        // Every wave collects its own entries, and each entry's place in its
+        // target's bucket follows from the wave, slot and lane that hold it
+        // (det_stable_add).  If a wave holds more than fits in registers,
         // the windows below do the step instead.
-        if (G <= 2) {  // uniform
+        if (G <= 2 && L < kDetLocalSlice) {  // uniform
             __shared__ int s_fit[1024 / kWave];
             char *wl = (char *)dyn + det_local_offset((size_t)(w2 - (char *)dyn) + (size_t)Wd * 6 + (size_t)L * 8);
-            uint32_t *dkey32 = (uint32_t *)(wl + (size_t)nwaves * kDetMap);
-            float *dw32 = (float *)(wl + (size_t)nwaves * kDetMap * 5);
+            uint8_t *scr8 = (uint8_t *)(wl + (size_t)nwaves * kDetMap);
+            uint16_t *cnt16 = (uint16_t *)(scr8 + det_counts_offset(L, nwaves));
+            float *dw32 = (float *)(wl + det_weights_offset(L, nwaves));
+            for (int i = tid; i < nwaves * L / 2; i += B) ((uint32_t *)cnt16)[i] = 0u;
             uint2 x[kDetSlots];
-            uint32_t key[kDetSlots];
-            const bool fits = wave_collect<kCoh, kSPT>(d, p, kept0, kept1, g0, (uint32_t *)segp + wave * kWave * kSPT,
-                                                       base + wave * kWave * kSPT,
-                                                       (uint8_t *)wl + (size_t)wave * kDetMap, x, key);
-            if (lane == 0) s_fit[wave] = fits ? 1 : 0;
-            __syncthreads();  // also orders the cursors' clearing before the counts
+            // vote: 0 = does not fit, 1 = no entries, 2 = entries (their
+            // loads stay in flight across the barrier)
+            const int fit = wave_collect<kCoh, kSPT>(d, p, kept0, kept1, g0, (uint32_t *)segp + wave * kWave * kSPT,
+                                                     base + wave * kWave * kSPT, (uint8_t *)wl + (size_t)wave * kDetMap,
+                                                     x);
+            if (lane == 0) s_fit[wave] = fit;
+            __syncthreads();  // also orders the clearing of cursors and counts before their use
             const bool all_fit = __ballot(lane < nwaves && s_fit[lane] == 0) == 0;  // uniform
             if (all_fit) {
-#pragma unroll
-                for (int i = 0; i < kDetSlots; ++i)
-                    if (x[i].x != 0xFFFFFFFFu) atomicAdd(&dcur[(int)x[i].x - lo], 1);
-                __syncthreads();
-                det_sort_add<kDetSlots>(acc, lo, x, key, dcur, dstart, dkey32, dw32, scratch, tb, te);
+                if (__ballot(lane < nwaves && s_fit[lane] == 2) != 0)  // uniform: some input this step
+                    det_stable_add<kDetSlots>(acc, lo, L, x, scr8, cnt16, dstart, dcur, dw32, scratch, tb, te);
                 return;
             }
         }
