@@ -721,11 +721,8 @@ __host__ __device__ inline size_t det_local_offset(size_t det_end) { return (det
 // Det, wave-local: slices shorter than this (the per-(wave, target) bytes
 // and counts grow with the slice; longer slices take the windows).
 constexpr int kDetLocalSlice = 1024;
-__host__ __device__ inline size_t det_counts_offset(int L, int nw) { return ((size_t)nw * L + 15) & ~(size_t)15; }
-__host__ __device__ inline size_t det_weights_offset(int L, int nw)
-{
-    return (size_t)nw * kDetMap + ((det_counts_offset(L, nw) + (size_t)nw * L * 2 + 15) & ~(size_t)15);
-}
+// The stable placement's 48 bytes per target follow the maps, then the weights.
+__host__ __device__ inline size_t det_weights_offset(int L, int nw) { return (size_t)nw * kDetMap + (size_t)L * 48; }
 __host__ __device__ inline int acc_floats(int L) { return (L + 3) & ~3; }
 __host__ __device__ inline size_t scan_bytes(int block) { return ((size_t)(block / 64 + 1) * 4 + 15) & ~(size_t)15; }
 __host__ __device__ inline size_t slice_lds_bytes(int L, int block, bool det, int ept, int spt, bool owner)
@@ -1068,31 +1065,37 @@ __device__ __forceinline__ void bucket_sums(float *acc, const float *dw, const i
 // its target's bucket is the number of that target's entries in earlier
 // waves plus those before it in this wave (earlier slots, then lower lanes
 // of its own slot).  Per slot, the lanes that share a target find each other
-// through a wave-private byte per target (each stores its lane id, all read
-// back the same one of them) and ballots; a wave-private count per target
-// carries the place across slots.  scr (u8) and cnt (u16, zero on entry)
-// hold nwaves x L, wave-major; dstart and dend get the buckets' bounds.
-// Ends with a workgroup barrier.
+// through a byte per (wave, target) (each stores its lane id, all read back
+// the same one of them) and ballots; a count per (wave, target) carries the
+// place across slots.
+// reg (48*L bytes, 16-byte aligned): per target a row of 16 bytes in each of
+// two byte arrays -- lane ids at 0, counts (zero on entry) at 32*L -- wave w
+// at byte (w + j) & 15 of row j, so that a wave's accesses to different
+// targets spread over the LDS banks and a target's counts are one 16-byte
+// load.  After the scan, per target a row of 16 u16 places (the same
+// rotation) at 0, over the lane ids and the 16*L bytes after them.  dstart
+// and dend get the buckets' bounds.  Ends with a workgroup barrier.
 template <int K>
-__device__ __forceinline__ void det_stable_add(float *acc, int lo, int L, const uint2 (&x)[K], uint8_t *scr,
-                                               uint16_t *cnt, int *dstart, int *dend, float *dw, int *scratch,
-                                               int tb, int te)
+__device__ __forceinline__ void det_stable_add(float *acc, int lo, const uint2 (&x)[K], uint8_t *reg, int L,
+                                               int *dstart, int *dend, float *dw, int *scratch, int tb, int te)
 {
-    const int lane = lane_id(), wave = wave_index(), nw = (int)blockDim.x / kWave;
-    uint8_t *sw = scr + wave * L;
-    uint16_t *cw = cnt + wave * L;
+    const int lane = lane_id(), wave = wave_index();
+    uint8_t *rep8 = reg;
+    uint8_t *cnt8 = reg + 32 * L;
+    uint16_t *off16 = (uint16_t *)reg;
     int rk[K];
 #pragma unroll
     for (int i = 0; i < K; ++i) {
         __builtin_amdgcn_sched_barrier(0);  // one slot at a time
         const bool valid = x[i].x != 0xFFFFFFFFu;
         const int j = valid ? (int)x[i].x - lo : 0;
-        if (valid) sw[j] = (uint8_t)lane;
+        const int a = 16 * j + ((wave + j) & 15);
+        if (valid) rep8[a] = (uint8_t)lane;
         // the byte read back may be another lane's store (the wave-level
         // sync also keeps the compiler from reusing this lane's own value)
         wave_sync();
-        const int rep = valid ? (int)sw[j] : -1;
-        const int prev = valid ? (int)cw[j] : 0;
+        const int rep = valid ? (int)rep8[a] : -1;
+        const int prev = valid ? (int)cnt8[a] : 0;
         uint64_t mine = valid ? (1ull << lane) : 0ull;
         uint64_t pend = __ballot(valid && rep != lane);  // lanes whose target another lane shares
         while (pend) {  // uniform: one pass per shared target
@@ -1103,44 +1106,74 @@ __device__ __forceinline__ void det_stable_add(float *acc, int lo, int L, const 
         }
         const int below = bits_below(mine), n = __popcll(mine);
         rk[i] = prev + below;
-        if (valid && below == n - 1) cw[j] = (uint16_t)(prev + n);
+        if (valid && below == n - 1) cnt8[a] = (uint8_t)(prev + n);  // a wave holds at most 128 spikes
     }
     __syncthreads();
-    // buckets: all waves' counts, a scan over the targets, then each wave's
-    // first place in the bucket (in place of its count); the counts of a
-    // target are read four waves at a time
+    // buckets: each target's total, a scan over the targets, then each
+    // wave's first place in the bucket
+    auto bytes4 = [](uint32_t v) {
+        const uint32_t t = (v & 0x00FF00FFu) + ((v >> 8) & 0x00FF00FFu);
+        return (int)((t & 0xFFFFu) + (t >> 16));
+    };
     int csum = 0;
     for (int j = tb; j < te; ++j) {
-        const uint16_t *cj = cnt + j;
-        for (int w0 = 0; w0 < nw; w0 += 4) {
-            int c[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) c[k] = w0 + k < nw ? (int)cj[(w0 + k) * L] : 0;
-            csum += (c[0] + c[1]) + (c[2] + c[3]);
-        }
+        const uint4 row = *(const uint4 *)(cnt8 + 16 * j);
+        csum += (bytes4(row.x) + bytes4(row.y)) + (bytes4(row.z) + bytes4(row.w));
     }
     int all = 0;
     int st = block_exclusive_scan(csum, scratch, &all);
     for (int j = tb; j < te; ++j) {
-        uint16_t *cj = cnt + j;
+        const uint4 row = *(const uint4 *)(cnt8 + 16 * j);
+        // rotate the row right by (j & 15) bytes: byte w is then wave w's count
+        const int q = (j >> 2) & 3, sh = j & 3;
+        const uint32_t r4[4] = {row.x, row.y, row.z, row.w};
+        uint32_t dq[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            dq[k] = q == 0 ? r4[k] : q == 1 ? r4[(k + 1) & 3] : q == 2 ? r4[(k + 2) & 3] : r4[(k + 3) & 3];
+        uint32_t c[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) c[k] = __builtin_amdgcn_alignbyte(dq[(k + 1) & 3], dq[k], (uint32_t)sh);
         dstart[j] = st;
-        for (int w0 = 0; w0 < nw; w0 += 4) {
-            int c[4];
+        // the places in wave order, two per register, then rotated left by
+        // (j & 15) u16 back to the row's order and stored as two 16-byte words
+        uint32_t pl[8];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) c[k] = w0 + k < nw ? (int)cj[(w0 + k) * L] : 0;
-#pragma unroll
-            for (int k = 0; k < 4; ++k)
-                if (w0 + k < nw) {
-                    cj[(w0 + k) * L] = (uint16_t)st;
-                    st += c[k];
-                }
+        for (int k = 0; k < 8; ++k) {
+            const uint32_t a = (uint32_t)st;
+            st += (int)((c[k >> 1] >> (16 * (k & 1))) & 0xFFu);
+            pl[k] = a | ((uint32_t)st << 16);
+            st += (int)((c[k >> 1] >> (16 * (k & 1) + 8)) & 0xFFu);
         }
+        const int h = (j >> 1) & 7;
+#pragma unroll
+        for (int b = 1; b < 8; b <<= 1)
+            if (h & b) {  // rotate the eight words up by b
+                uint32_t t[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) t[k] = pl[(k - b) & 7];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) pl[k] = t[k];
+            }
+        if (j & 1) {
+            uint32_t t[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) t[k] = __builtin_amdgcn_alignbit(pl[k], pl[(k + 7) & 7], 16u);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) pl[k] = t[k];
+        }
+        uint4 *o = (uint4 *)(off16 + 16 * j);
+        o[0] = make_uint4(pl[0], pl[1], pl[2], pl[3]);
+        o[1] = make_uint4(pl[4], pl[5], pl[6], pl[7]);
         dend[j] = st;
     }
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < K; ++i)
-        if (x[i].x != 0xFFFFFFFFu) dw[(int)cw[(int)x[i].x - lo] + rk[i]] = u2f(x[i].y);
+        if (x[i].x != 0xFFFFFFFFu) {
+            const int j = (int)x[i].x - lo;
+            dw[(int)off16[16 * j + ((wave + j) & 15)] + rk[i]] = u2f(x[i].y);
+        }
     __syncthreads();
     bucket_sums(acc, dw, dstart, dend, tb, te);
 }
@@ -1258,10 +1291,9 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
         if (G <= 2 && L < kDetLocalSlice) {  // uniform
             __shared__ int s_fit[1024 / kWave];
             char *wl = (char *)dyn + det_local_offset((size_t)(w2 - (char *)dyn) + (size_t)Wd * 6 + (size_t)L * 8);
-            uint8_t *scr8 = (uint8_t *)(wl + (size_t)nwaves * kDetMap);
-            uint16_t *cnt16 = (uint16_t *)(scr8 + det_counts_offset(L, nwaves));
+            uint8_t *reg = (uint8_t *)(wl + (size_t)nwaves * kDetMap);
             float *dw32 = (float *)(wl + det_weights_offset(L, nwaves));
-            for (int i = tid; i < nwaves * L / 2; i += B) ((uint32_t *)cnt16)[i] = 0u;
+            for (int i = tid; i < 4 * L; i += B) ((uint32_t *)(reg + 32 * L))[i] = 0u;  // the counts
             uint2 x[kDetSlots];
             // vote: 0 = does not fit, 1 = no entries, 2 = entries (their
             // loads stay in flight across the barrier)
@@ -1273,7 +1305,7 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
             const bool all_fit = __ballot(lane < nwaves && s_fit[lane] == 0) == 0;  // uniform
             if (all_fit) {
                 if (__ballot(lane < nwaves && s_fit[lane] == 2) != 0)  // uniform: some input this step
-                    det_stable_add<kDetSlots>(acc, lo, L, x, scr8, cnt16, dstart, dcur, dw32, scratch, tb, te);
+                    det_stable_add<kDetSlots>(acc, lo, x, reg, L, dstart, dcur, dw32, scratch, tb, te);
                 return;
             }
         }
