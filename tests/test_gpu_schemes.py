@@ -325,6 +325,29 @@ def test_slices_of_odd_word_pairs(g4s, oracle, block, slices):
             assert np.array_equal(dense(g4s, words, n), oraster[:100]), (alg, block)
 
 
+@pytest.mark.parametrize("block", [512, 1024])
+def test_det_slices_around_the_wave_local_limit(g4s, oracle, block):
+    """Det slices just under 1024 neurons (960: the wave-local stable
+    placement with its largest per-target rows) and over it (1504: ordered
+    windows), at N=30,000 with 500 synapses per row (about 16 entries per
+    spike and slice).  With the thalamic drive at 2.5x, busy steps overflow a
+    wave's 384 register entries and take the windows while quiet ones stay
+    wave-local, so one run mixes both.  Bit-identical to the oracle."""
+    n, nsyn, seed, steps = 30_000, 500, 17, 200
+    for thal in (1.0, 2.5):
+        onet = oracle.Network(n, nsyn, seed, regime=1, thal_scale=thal)
+        oraster, ocounts, _ = onet.run(steps, threads=8)
+        for slices in (32, 20):
+            for alg in ("AB", "SKL"):
+                with g4s.Network(n, nsyn, seed, regime=1, thal_scale=thal) as net:
+                    words, counts = net.run(steps, alg=alg, accum="det", slices=slices, block=block)
+                    assert net.timing()["grid"] == slices, (alg, slices)
+                    assert np.array_equal(counts, ocounts), (alg, block, slices, thal)
+                    assert np.array_equal(g4s.raster_to_dense(words, n), oraster), (alg, block, slices, thal)
+                    v, _, _, _ = net.state()
+                    assert np.array_equal(v.view(np.uint32), onet.v.view(np.uint32)), (alg, block, slices, thal)
+
+
 @pytest.mark.parametrize("block", [256, 1024])
 def test_large_slices_both_modes(g4s, oracle, block):
     """16 slices of 3136 neurons at N=50,000.  Atomic mode runs the large
