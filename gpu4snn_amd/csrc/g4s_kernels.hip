@@ -1178,6 +1178,93 @@ __device__ __forceinline__ void det_stable_add(float *acc, int lo, const uint2 (
     bucket_sums(acc, dw, dstart, dend, tb, te);
 }
 
+// Det mode, one window of a four-wave workgroup whose wave w holds window
+// entries (w*K + i)*64 + lane in x[i] (targets j + lo, 0xFFFFFFFF for none):
+// acc[j] <- acc[j] plus the window's entries for j in window order, which is
+// (wave, i, lane).  Placement as in det_stable_add, with one byte per (wave,
+// target): the lanes' ids in dw (free until the weights are placed; 4*len
+// bytes), the counts in dcur[j] (zero on entry, byte w for wave w).  The scan
+// then leaves in dstart[j] and dcur[j] the first places of waves 0, 1 and 2,
+// 3 (u16 each), and target j's bucket ends where target j+1's starts.
+// Leaves dcur zero; the caller ends the window with a workgroup barrier.
+// A count beyond a byte (256 of a wave's entries for one target: segments
+// of about one entry, all to one neuron) is flagged instead: the call then
+// clears the counts and returns false, and the caller sorts the window.
+// ovf: an LDS flag, zero on entry and on return.
+template <int K>
+__device__ __forceinline__ bool det_window_stable(float *acc, int lo, int len, const uint2 (&x)[K], float *dw,
+                                                  int *dstart, int *dcur, int *scratch, int tb, int te, int *ovf)
+{
+    const int lane = lane_id(), wave = wave_index();
+    uint8_t *rep8 = (uint8_t *)dw;
+    uint8_t *cnt8 = (uint8_t *)dcur;
+    int rk[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+        __builtin_amdgcn_sched_barrier(0);  // one slot at a time
+        const bool valid = x[i].x != 0xFFFFFFFFu;
+        const int a = valid ? 4 * ((int)x[i].x - lo) + wave : 0;
+        if (valid) rep8[a] = (uint8_t)lane;
+        wave_sync();  // another lane's id may be read back (and this lane's own is not reused)
+        const int rep = valid ? (int)rep8[a] : -1;
+        const int prev = valid ? (int)cnt8[a] : 0;
+        uint64_t mine = valid ? (1ull << lane) : 0ull;
+        uint64_t pend = __ballot(valid && rep != lane);
+        while (pend) {  // uniform: one pass per shared target
+            const int r = __builtin_amdgcn_readlane(rep, (int)__builtin_ctzll(pend));
+            const uint64_t grp = __ballot(valid && rep == r);
+            if (valid && rep == r) mine = grp;
+            pend &= ~grp;
+        }
+        const int below = bits_below(mine), n = __popcll(mine);
+        rk[i] = prev + below;
+        if (valid && below == n - 1) {
+            cnt8[a] = (uint8_t)(prev + n);
+            if (prev + n > 255) *ovf = 1;
+        }
+    }
+    __syncthreads();
+    if (*ovf) {  // uniform
+        for (int j = tb; j < te; ++j) dcur[j] = 0;
+        __syncthreads();  // every thread has read the flag
+        if (threadIdx.x == 0) *ovf = 0;
+        __syncthreads();
+        return false;
+    }
+    int csum = 0;
+    for (int j = tb; j < te; ++j) {
+        const uint32_t c = (uint32_t)dcur[j];
+        csum += (int)((c & 0xFFu) + ((c >> 8) & 0xFFu) + ((c >> 16) & 0xFFu) + (c >> 24));
+    }
+    int nwin = 0;
+    int st = block_exclusive_scan(csum, scratch, &nwin);
+    for (int j = tb; j < te; ++j) {
+        const uint32_t c = (uint32_t)dcur[j];
+        const uint32_t p0 = (uint32_t)st, p1 = p0 + (c & 0xFFu), p2 = p1 + ((c >> 8) & 0xFFu), p3 = p2 + ((c >> 16) & 0xFFu);
+        st = (int)(p3 + (c >> 24));
+        dstart[j] = (int)(p0 | (p1 << 16));
+        dcur[j] = (int)(p2 | (p3 << 16));
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < K; ++i)
+        if (x[i].x != 0xFFFFFFFFu) {
+            const int j = (int)x[i].x - lo;
+            const uint32_t pw = (uint32_t)(wave < 2 ? dstart[j] : dcur[j]);
+            dw[(int)((pw >> (16 * (wave & 1))) & 0xFFFFu) + rk[i]] = u2f(x[i].y);
+        }
+    __syncthreads();
+    for (int j = tb; j < te; ++j) {
+        const int s0 = dstart[j] & 0xFFFF;
+        const int s1 = j + 1 < len ? (dstart[j + 1] & 0xFFFF) : nwin;
+        float a = acc[j];
+        for (int q = s0; q < s1; ++q) a = a + dw[q];
+        acc[j] = a;
+        dcur[j] = 0;
+    }
+    return true;
+}
+
 // acc[0..len) <- input of slice p from the rows that spiked at step t-1.
 // Three dependent global levels, each with all its loads in flight:
 //  1. the step's bitmap, compacted in LDS into the ascending spike list
@@ -1232,6 +1319,8 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
     uint16_t *dkey = (uint16_t *)(w2 + (size_t)Wd * 4);  // det: also the owner map
     int *dcur = (int *)(w2 + (size_t)Wd * 6);
     int *dstart = dcur + L;
+    __shared__ int s_ovf;  // det windows: a wave's count beyond a byte (det_window_stable)
+    if (kDet && tid == 0) s_ovf = 0;
     if (!kZeroed)
         for (int i = tid; i < len; i += B) {
             acc[i] = 0.0f;
@@ -1492,6 +1581,14 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
         // contributions reach the LDS atomics nearly in ascending source
         // order, which keeps atomic mode on the ordered sum for many steps.
         uint32_t carry = 0;
+        // det with four waves and slices of at most Wd targets: each wave
+        // takes a contiguous quarter of the window (entry f0 + (w*kEPT + k)*64
+        // + lane in its slot k), so that the order (wave, slot, lane) is the
+        // window's order and the entries are placed without a sort
+        // (det_window_stable); otherwise entry f0 + k*B + tid
+        const bool stable_win = kDet && B == 4 * kWave && L <= Wd;  // uniform
+        const int wv = wave_index(), ln = lane_id();
+        auto off_of = [&](int k) { return stable_win ? (wv * kEPT + k) * kWave + ln : k * B + tid; };
         for (int f0 = 0, win = 0; f0 < E; f0 += Wd, ++win) {
             int sa[kEPT];
             if (kOwner) {
@@ -1505,7 +1602,7 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
                 if (!kDet) clear_owner<kEPT>((win & 1) ? own0 : own1);
                 owner_scan<kEPT>(own, scratch, &carry);
 #pragma unroll
-                for (int k = 0; k < kEPT; ++k) sa[k] = own[k * B + tid];
+                for (int k = 0; k < kEPT; ++k) sa[k] = own[off_of(k)];
             } else {
                 // the last spike a < m with segp[a] <= f, which holds entry f
                 // (segp ascending from segp[0] = 0; empty segments repeat
@@ -1522,21 +1619,25 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
                         v[k] = segp[c[k]];
                     }
 #pragma unroll
-                    for (int k = 0; k < kEPT; ++k) sa[k] = v[k] <= f0 + k * B + tid ? c[k] : sa[k];
+                    for (int k = 0; k < kEPT; ++k) sa[k] = v[k] <= f0 + off_of(k) ? c[k] : sa[k];
                 }
             }
             // every lookup (sa < m) before the first load
             uint64_t idx[kEPT];
 #pragma unroll
-            for (int k = 0; k < kEPT; ++k) idx[k] = base[sa[k]] + (uint64_t)(f0 + k * B + tid);
+            for (int k = 0; k < kEPT; ++k) idx[k] = base[sa[k]] + (uint64_t)(f0 + off_of(k));
             uint2 x[kEPT];
 #pragma unroll
             for (int k = 0; k < kEPT; ++k) {
                 x[k] = make_uint2(0xFFFFFFFFu, 0u);
-                if (f0 + k * B + tid < E) x[k] = entry_at(d.syn, (size_t)idx[k]);
+                if (f0 + off_of(k) < E) x[k] = entry_at(d.syn, (size_t)idx[k]);
             }
             if (!kDet) {
                 lds_add_cas<kEPT>(acc, x, lo);
+            } else if (stable_win &&
+                       det_window_stable<kEPT>(acc, lo, len, x, dw, dstart, dcur, scratch, tb, te, &s_ovf)) {
+                if (kOwner) clear_owner<kEPT>(own0);
+                __syncthreads();
             } else {
                 // ordered: counting sort of the window by target, then each
                 // target adds its entries in window-offset (= source) order
