@@ -1031,8 +1031,11 @@ __device__ __forceinline__ int wave_collect(const Net &d, int p, uint4 kept0, ui
 // Det mode, last phase: each target j in [tb, te) adds its bucket
 // dw[dstart[j] .. dend[j]) in order, the weights read 16 at a time (reads
 // may pass the bucket's ends, within the workspace); dend is left zero.
-// Ends with a workgroup barrier.
-__device__ __forceinline__ void bucket_sums(float *acc, const float *dw, const int *dstart, int *dend, int tb, int te)
+// Ends with a workgroup barrier unless `own` is set: then every thread owns
+// at most one target, the one its update reads (j = tid), and nothing else
+// of the call is read before the caller's next barrier.
+__device__ __forceinline__ void bucket_sums(float *acc, const float *dw, const int *dstart, int *dend, int tb, int te,
+                                            bool own = false)
 {
     for (int j = tb; j < te; ++j) {
         const int b0 = dstart[j], b1 = dend[j];
@@ -1054,7 +1057,7 @@ __device__ __forceinline__ void bucket_sums(float *acc, const float *dw, const i
         acc[j] = a;
         dend[j] = 0;
     }
-    __syncthreads();
+    if (!own) __syncthreads();
 }
 
 // Det mode, stable placement: acc[j] <- acc[j] plus the entries of every
@@ -1074,7 +1077,9 @@ __device__ __forceinline__ void bucket_sums(float *acc, const float *dw, const i
 // targets spread over the LDS banks and a target's counts are one 16-byte
 // load.  After the scan, per target a row of 16 u16 places (the same
 // rotation) at 0, over the lane ids and the 16*L bytes after them.  dstart
-// and dend get the buckets' bounds.  Ends with a workgroup barrier.
+// and dend get the buckets' bounds.  Ends with a workgroup barrier if the
+// slice is longer than the workgroup; otherwise thread j has summed target j,
+// the neuron its update reads, and the caller's next barrier orders the rest.
 template <int K>
 __device__ __forceinline__ void det_stable_add(float *acc, int lo, const uint2 (&x)[K], uint8_t *reg, int L,
                                                int *dstart, int *dend, float *dw, int *scratch, int tb, int te)
@@ -1175,7 +1180,7 @@ __device__ __forceinline__ void det_stable_add(float *acc, int lo, const uint2 (
             dw[(int)off16[16 * j + ((wave + j) & 15)] + rk[i]] = u2f(x[i].y);
         }
     __syncthreads();
-    bucket_sums(acc, dw, dstart, dend, tb, te);
+    bucket_sums(acc, dw, dstart, dend, tb, te, L <= (int)blockDim.x);
 }
 
 // Det mode, one window of a four-wave workgroup whose wave w holds window
@@ -1277,7 +1282,9 @@ __device__ __forceinline__ bool det_window_stable(float *acc, int lo, int len, c
 //     search over the segment offsets names each entry's spike, and one
 //     table load per entry follows.
 // Spikes beyond kSPT*B are handled in further passes.  dyn is the LDS
-// workspace (acc first).  Ends with a workgroup barrier.
+// workspace (acc first).  Ends with a workgroup barrier, except after the
+// det wave-local sums of a slice no longer than the workgroup, where each
+// thread's acc[tid] is its own sum (the update reads only that).
 // kDet = false: binary32 adds applied atomically (compare-and-swap), any order.
 // kDet = true: every acc[j] is summed in ascending presynaptic id from 0.0f,
 // bit-equal to a sequential push.  The entries of a pass are in ascending
