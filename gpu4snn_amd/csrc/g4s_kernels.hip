@@ -1028,36 +1028,36 @@ __device__ __forceinline__ int wave_collect(const Net &d, int p, uint4 kept0, ui
     return 2;
 }
 
-// Det mode, last phase: each target j in [tb, te) adds its bucket
-// dw[dstart[j] .. dend[j]) in order, the weights read 16 at a time (reads
-// may pass the bucket's ends, within the workspace); dend is left zero.
-// Ends with a workgroup barrier unless `own` is set: then every thread owns
-// at most one target, the one its update reads (j = tid), and nothing else
-// of the call is read before the caller's next barrier.
-__device__ __forceinline__ void bucket_sums(float *acc, const float *dw, const int *dstart, int *dend, int tb, int te,
-                                            bool own = false)
+// Det mode, last phase: the bucket dw[b0 .. b1) added in order to a, the
+// weights read 16 at a time (reads may pass the bucket's ends, within the
+// workspace).
+__device__ __forceinline__ float bucket_sum(const float *dw, int b0, int b1, float a)
 {
-    for (int j = tb; j < te; ++j) {
-        const int b0 = dstart[j], b1 = dend[j];
-        float a = acc[j];
-        for (int q0 = b0 & ~3; q0 < b1; q0 += 16) {
-            float4 wv[4];
+    for (int q0 = b0 & ~3; q0 < b1; q0 += 16) {
+        float4 wv[4];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) wv[c] = *(const float4 *)(dw + q0 + 4 * c);
+        for (int c = 0; c < 4; ++c) wv[c] = *(const float4 *)(dw + q0 + 4 * c);
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const float h[4] = {wv[c].x, wv[c].y, wv[c].z, wv[c].w};
+        for (int c = 0; c < 4; ++c) {
+            const float h[4] = {wv[c].x, wv[c].y, wv[c].z, wv[c].w};
 #pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const int q = q0 + 4 * c + e;
-                    if (q >= b0 && q < b1) a = a + h[e];
-                }
+            for (int e = 0; e < 4; ++e) {
+                const int q = q0 + 4 * c + e;
+                if (q >= b0 && q < b1) a = a + h[e];
             }
         }
-        acc[j] = a;
+    }
+    return a;
+}
+// Each target j in [tb, te) adds its bucket dw[dstart[j] .. dend[j]); dend
+// is left zero.  Ends with a workgroup barrier.
+__device__ __forceinline__ void bucket_sums(float *acc, const float *dw, const int *dstart, int *dend, int tb, int te)
+{
+    for (int j = tb; j < te; ++j) {
+        acc[j] = bucket_sum(dw, dstart[j], dend[j], acc[j]);
         dend[j] = 0;
     }
-    if (!own) __syncthreads();
+    __syncthreads();
 }
 
 // Det mode, stable placement: acc[j] <- acc[j] plus the entries of every
@@ -1127,6 +1127,11 @@ __device__ __forceinline__ void det_stable_add(float *acc, int lo, const uint2 (
     }
     int all = 0;
     int st = block_exclusive_scan(csum, scratch, &all);
+    // a slice no longer than the workgroup: thread j's one target is j, its
+    // bucket bounds stay in registers and its input starts from 0.0f (acc
+    // is cleared on entry to the accumulation)
+    const bool own = L <= (int)blockDim.x;  // uniform
+    int b0own = 0, b1own = 0;
     for (int j = tb; j < te; ++j) {
         const uint4 row = *(const uint4 *)(cnt8 + 16 * j);
         // rotate the row right by (j & 15) bytes: byte w is then wave w's count
@@ -1139,7 +1144,8 @@ __device__ __forceinline__ void det_stable_add(float *acc, int lo, const uint2 (
         uint32_t c[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) c[k] = __builtin_amdgcn_alignbyte(dq[(k + 1) & 3], dq[k], (uint32_t)sh);
-        dstart[j] = st;
+        if (own) b0own = st;
+        else dstart[j] = st;
         // the places in wave order, two per register, then rotated left by
         // (j & 15) u16 back to the row's order and stored as two 16-byte words
         uint32_t pl[8];
@@ -1170,7 +1176,8 @@ __device__ __forceinline__ void det_stable_add(float *acc, int lo, const uint2 (
         uint4 *o = (uint4 *)(off16 + 16 * j);
         o[0] = make_uint4(pl[0], pl[1], pl[2], pl[3]);
         o[1] = make_uint4(pl[4], pl[5], pl[6], pl[7]);
-        dend[j] = st;
+        if (own) b1own = st;
+        else dend[j] = st;
     }
     __syncthreads();
 #pragma unroll
@@ -1180,7 +1187,11 @@ __device__ __forceinline__ void det_stable_add(float *acc, int lo, const uint2 (
             dw[(int)off16[16 * j + ((wave + j) & 15)] + rk[i]] = u2f(x[i].y);
         }
     __syncthreads();
-    bucket_sums(acc, dw, dstart, dend, tb, te, L <= (int)blockDim.x);
+    if (own) {
+        if (tb < te) acc[tb] = bucket_sum(dw, b0own, b1own, 0.0f);
+    } else {
+        bucket_sums(acc, dw, dstart, dend, tb, te);
+    }
 }
 
 // Det mode, one window of a four-wave workgroup whose wave w holds window
