@@ -27,7 +27,8 @@ constexpr int kCtlStride = 4;                 // per-step control words
 // workgroups in word 1), [2..3] the step's start on the device wall clock.
 enum { kCtlCount = 0, kCtlStamp = 2 };
 // A run's control block: kCtlHead words (word 0 the error flag of bounded
-// waits) followed by the control rows, so that one copy brings both back.
+// waits) followed by the control rows; the chunk-end kernel writes both to
+// the host in one pass.
 constexpr int kCtlHead = 4;
 constexpr uint32_t kSpinLimit = 1u << 24;     // bounded device waits (~10 s)
 // Grid barrier: one arrival counter and one release flag per group of
