@@ -2028,6 +2028,9 @@ __global__ void __launch_bounds__(1024) k_skl_slice(DevNet d, int t0, int steps,
 {
     extern __shared__ __align__(16) float dyn[];  // slice_accumulate needs 16-byte alignment
     __shared__ int s_wcnt[1024 / kWave];  // per-wave spike totals of the step
+    // the first step's start stamp is the launch's start (its state loads
+    // belong to the chunk's time); later steps stamp their own start
+    if (blockIdx.x == 0 && threadIdx.x == 0) stamp_step(d.ctl, 0);
     float *acc = dyn;
     const int p = slice_of_block((int)blockIdx.x, (int)gridDim.x);
     const int lo = p * d.L;
@@ -2059,7 +2062,7 @@ __global__ void __launch_bounds__(1024) k_skl_slice(DevNet d, int t0, int steps,
     }
     for (int rel = 0; rel < steps; ++rel) {
         const int t = t0 + rel;
-        if (blockIdx.x == 0 && tid == 0) stamp_step(d.ctl, rel);
+        if (blockIdx.x == 0 && tid == 0 && rel > 0) stamp_step(d.ctl, rel);
         // det mode computes the step's thalamic draws while the accumulation's
         // first bitmap loads are in flight (18.25 -> 18.0 us per step at
         // N=100k); in atomic mode the draws follow the accumulation, which

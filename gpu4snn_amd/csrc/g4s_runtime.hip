@@ -795,12 +795,27 @@ int run_impl(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, 
     Pending pend[2];
     int npend = 0;
 
+    // A chunk's device time: for SKL, one launch whose workgroup 0 stamps
+    // each step's start, the stamps (first step to the chunk's end stamp)
+    // instead of event markers around the launch, which cost each run a few
+    // microseconds; other schemes and runs with kernel events use the markers.
+    const bool stamp_time = alg == G4S_ALG_SKL && !net->kev_on && net->clock_khz > 0;
+    auto ctl_stamp = [](const int32_t *rows, int r) {
+        return (uint64_t)(uint32_t)rows[r * kCtlStride + kCtlStamp] |
+               ((uint64_t)(uint32_t)rows[r * kCtlStride + kCtlStamp + 1] << 32);
+    };
     // consume the oldest enqueued chunk
     auto finish = [&](const Pending &pc) -> int {
         HIP_OK(hipEventSynchronize(net->done_ev[pc.buf]));
         if (net->h_ctl[pc.buf][0]) return fail(G4S_ERR_TIMEOUT, "a bounded device wait expired at chunk starting step %d", pc.t0);
         float ms = 0.0f;
-        HIP_OK(hipEventElapsedTime(&ms, net->ev0[pc.buf], net->ev1[pc.buf]));
+        if (stamp_time) {  // the chunk's first step start to its end stamp
+            ms = (float)((double)(int64_t)(ctl_stamp(net->h_ctl[pc.buf] + kCtlHead, pc.cs) -
+                                           ctl_stamp(net->h_ctl[pc.buf] + kCtlHead, 0)) /
+                         net->clock_khz);
+        } else {
+            HIP_OK(hipEventElapsedTime(&ms, net->ev0[pc.buf], net->ev1[pc.buf]));
+        }
         dev_ms += ms;
         if (net->kev_on) {
             kern_launches += net->kev_used / 2;
@@ -808,10 +823,7 @@ int run_impl(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, 
         }
         const int32_t *hc = net->h_ctl[pc.buf] + kCtlHead;
         int64_t mx = 0;
-        auto stamp = [&](int r) {
-            return (uint64_t)(uint32_t)hc[r * kCtlStride + kCtlStamp] |
-                   ((uint64_t)(uint32_t)hc[r * kCtlStride + kCtlStamp + 1] << 32);
-        };
+        auto stamp = [&](int r) { return ctl_stamp(hc, r); };
         for (int r = 0; r < pc.cs; ++r) {
             const int32_t cnt = hc[r * kCtlStride + kCtlCount];
             spikes += cnt;
@@ -856,7 +868,7 @@ int run_impl(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, 
             ti[1] = (int32_t)(t_end - 1);
             HIP_OK(hipMemcpyAsync((void *)d.tinfo, ti, 2 * sizeof(int32_t), hipMemcpyHostToDevice, net->stream));
         }
-        HIP_OK(hipEventRecord(net->ev0[buf], net->stream));
+        if (!stamp_time) HIP_OK(hipEventRecord(net->ev0[buf], net->stream));
         if (slice_steps) {
             if ((rc = run_ab_chunk(net, cs, use_graph, sblock, det))) return abandon(rc);
             launches += cs;
@@ -877,7 +889,7 @@ int run_impl(g4s_net *net, int alg, int accum, int32_t steps, uint32_t *raster, 
                 dom_grid = L.grid;
             }
         }
-        HIP_OK(hipEventRecord(net->ev1[buf], net->stream));
+        if (!stamp_time) HIP_OK(hipEventRecord(net->ev1[buf], net->stream));
         // one kernel: the chunk's end stamp, the step counts (summed over the
         // slices for slice schemes), the error flag and control rows written
         // straight to the host buffer, and the caller's counts.  Without a

@@ -71,7 +71,10 @@ typedef struct g4s_run_opts {
 
 /* Timing of the last g4s_run call. */
 typedef struct g4s_timing {
-    double  device_ms;   /* HIP-event time of the step loop, raster copies excluded    */
+    double  device_ms;   /* device time of the step loop, raster copies excluded: HIP  *
+                          * events around each chunk's launches; SKL without kernel    *
+                          * events: its launch's start to the chunk's end stamp (device *
+                          * wall clock)                                                */
     double  wall_ms;     /* host steady-clock time of the whole call                   */
     int64_t steps;       /* steps simulated                                            */
     int64_t spikes;      /* spikes in those steps                                      */
