@@ -90,6 +90,7 @@ struct Inst {
     g4s_spec spec{};
     DevNet d{};
     void *block = nullptr;
+    size_t block_bytes = 0;  // allocated size of block (a reused block may be larger)
     int64_t bytes = 0;      // device block + this instance's share of the pools
     int64_t t = 0, steps = 0, spikes = 0;
     bool keep_raster = false;
@@ -145,9 +146,80 @@ struct g4s_batch {
     size_t g_lds = 0;
     int64_t gen = 0, g_gen = -1;
     g4s_timing timing{};
+    // Device blocks of removed instances, kept for the next instances that
+    // fit them: hipFree takes about 0.25 ms and waits for the device, which
+    // a server admitting and retiring hundreds of requests per second pays
+    // on every request.  At most kSpareBytes in all.
+    std::vector<std::pair<void *, size_t>> spare;
+    size_t spare_bytes = 0;
 };
 
 namespace {
+
+constexpr size_t kSpareBytes = (size_t)16 << 30;
+
+void drop_spare(g4s_batch *b)
+{
+    for (auto &e : b->spare) (void)hipFree(e.first);
+    b->spare.clear();
+    b->spare_bytes = 0;
+}
+
+// A device block of at least `need` bytes: the smallest spare block no more
+// than an eighth larger, else a new allocation (after releasing the spare
+// blocks if the device is out of memory).  *got = its size; null on failure.
+void *take_block(g4s_batch *b, size_t need, size_t *got)
+{
+    size_t best = 0;
+    int k_best = -1;
+    for (size_t k = 0; k < b->spare.size(); ++k) {
+        const size_t have = b->spare[k].second;
+        if (have >= need && have - need <= need / 8 && (k_best < 0 || have < best)) {
+            best = have;
+            k_best = (int)k;
+        }
+    }
+    if (k_best >= 0) {
+        void *p = b->spare[(size_t)k_best].first;
+        b->spare.erase(b->spare.begin() + k_best);
+        b->spare_bytes -= best;
+        *got = best;
+        return p;
+    }
+    void *p = nullptr;
+    if (hipMalloc(&p, need) != hipSuccess) {
+        (void)hipGetLastError();  // the failed allocation's status is not a later launch's
+        if (b->spare.empty()) return nullptr;
+        drop_spare(b);
+        p = nullptr;
+        if (hipMalloc(&p, need) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+    }
+    *got = need;
+    return p;
+}
+
+// Keep a block no longer used by any queued work for reuse, releasing the
+// oldest spare blocks beyond kSpareBytes.
+void put_block(g4s_batch *b, void *p, size_t bytes)
+{
+    if (!p) return;
+    if (bytes > kSpareBytes / 4) {
+        (void)hipFree(p);
+        return;
+    }
+    size_t drop = 0;
+    while (drop < b->spare.size() && b->spare_bytes + bytes > kSpareBytes) {
+        (void)hipFree(b->spare[drop].first);
+        b->spare_bytes -= b->spare[drop].second;
+        ++drop;
+    }
+    b->spare.erase(b->spare.begin(), b->spare.begin() + (ptrdiff_t)drop);
+    b->spare.emplace_back(p, bytes);
+    b->spare_bytes += bytes;
+}
 
 template <class T>
 int grow_dev(g4s_batch *b, T **d, T **h, size_t *cap, size_t need)
@@ -330,9 +402,9 @@ int batch_add(g4s_batch *b, const g4s_spec *spec, int64_t steps, int keep_raster
                     (long long)b->bytes_in_use, (long long)b->opts.budget);
 
     const Layout lay = layout_of(n, spec->nsyn, P);
-    void *blk = nullptr;
-    if (hipMalloc(&blk, lay.total) != hipSuccess)
-        return fail(G4S_ERR_MEM, "hipMalloc(%zu) for an instance of N=%lld", lay.total, (long long)n);
+    size_t blk_bytes = 0;
+    void *blk = take_block(b, lay.total, &blk_bytes);
+    if (!blk) return fail(G4S_ERR_MEM, "hipMalloc(%zu) for an instance of N=%lld", lay.total, (long long)n);
     char *base = (char *)blk;
     DevNet d{};
     d.n = (int32_t)n;
@@ -360,7 +432,7 @@ int batch_add(g4s_batch *b, const g4s_spec *spec, int64_t steps, int keep_raster
     if (e == hipSuccess) e = launch_slice_offsets(d, soff, b->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
     if (e != hipSuccess) {
-        (void)hipFree(blk);
+        (void)hipFree(blk);  // not reused: the device may be in an error state
         return fail(G4S_ERR_HIP, "instance initialisation: %s", hipGetErrorString(e));
     }
     d.soff = soff;
@@ -381,6 +453,7 @@ int batch_add(g4s_batch *b, const g4s_spec *spec, int64_t steps, int keep_raster
     in.spec = *spec;
     in.d = d;
     in.block = blk;
+    in.block_bytes = blk_bytes;
     in.bytes = fp;
     in.steps = steps;
     in.keep_raster = keep_raster != 0;
@@ -718,8 +791,8 @@ int g4s_batch_remove(g4s_batch *b, int32_t slot)
     int rc = check_slot(b, slot);
     if (rc) return rc;
     Inst &in = b->inst[(size_t)slot];
-    HIP_OK(hipStreamSynchronize(b->stream));
-    if (in.block) (void)hipFree(in.block);
+    HIP_OK(hipStreamSynchronize(b->stream));  // no queued work still uses the block
+    put_block(b, in.block, in.block_bytes);
     b->bytes_in_use -= in.bytes;
     in = Inst();
     return G4S_OK;
@@ -976,6 +1049,7 @@ int g4s_batch_destroy(g4s_batch *b)
     drop_graph(b);
     for (Inst &in : b->inst)
         if (in.block) (void)hipFree(in.block);
+    drop_spare(b);
     for (void *p : {(void *)b->d_desc, (void *)b->d_work, (void *)b->d_ring, (void *)b->d_bcnt, (void *)b->d_slots,
                     (void *)b->d_counts, (void *)b->d_units, (void *)b->d_ctr, (void *)b->d_done})
         if (p) (void)hipFree(p);

@@ -12,7 +12,8 @@ int fail(int code, const char *fmt, ...);
 int validate_spec(const g4s_spec *s);
 int64_t n_exc_of(const g4s_spec *s);
 // Outgoing table of `spec` into device memory `syn`: built on the device
-// (Nsyn <= kDeviceBuildMaxNsyn) or on the host and uploaded; same table.
+// (Nsyn <= kDeviceBuildMaxNsyn; queued on `stream`, not waited for) or on
+// the host and uploaded; same table.
 int build_synapses(const g4s_spec *spec, uint2 *syn, hipStream_t stream);
 
 #define G4S_HIP_OK(expr)                                                                           \

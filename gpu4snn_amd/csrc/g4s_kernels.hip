@@ -2141,40 +2141,40 @@ __global__ void __launch_bounds__(1024) k_skl_slice(DevNet d, int t0, int steps,
 // ---------------------------------------------------------------------------
 constexpr int kDevRowMax = 1024;
 
-__device__ __forceinline__ void sort64_wave(uint64_t *k, int S)
+// Bitonic sort of k[0..S) in LDS by one 64-thread workgroup (S a power of
+// two, 64 <= S <= kDevRowMax).  A stage compares the pairs (i, i | j) with
+// bit j of i clear: pair q is i = q with a zero bit inserted at bit log2(j).
+// Lane l takes pairs l + 64r; it reads all of its pairs before it writes any
+// (the pairs of a stage are disjoint), so a stage waits for one LDS round
+// trip, not one per pair.
+template <class T>
+__device__ __forceinline__ void sort_row(T *k, int S)
 {
+    constexpr int kPairs = kDevRowMax / 128;
+    const int half = S >> 1;
+    const int lane = (int)threadIdx.x;
     for (int kk = 2; kk <= S; kk <<= 1)
         for (int j = kk >> 1; j > 0; j >>= 1) {
-            for (int i = (int)threadIdx.x; i < S; i += 64) {
-                const int l = i ^ j;
-                if (l > i) {
-                    const uint64_t a = k[i], b = k[l];
-                    const bool up = (i & kk) == 0;
-                    if ((a > b) == up) {
-                        k[i] = b;
-                        k[l] = a;
-                    }
+            T a[kPairs], b[kPairs];
+            int ia[kPairs];
+#pragma unroll
+            for (int r = 0; r < kPairs; ++r) {
+                const int q = lane + 64 * r;
+                ia[r] = ((q & ~(j - 1)) << 1) | (q & (j - 1));
+                if (q < half) {
+                    a[r] = k[ia[r]];
+                    b[r] = k[ia[r] | j];
                 }
             }
-            __syncthreads();
-        }
-}
-
-__device__ __forceinline__ void sort32_wave(uint32_t *k, int S)
-{
-    for (int kk = 2; kk <= S; kk <<= 1)
-        for (int j = kk >> 1; j > 0; j >>= 1) {
-            for (int i = (int)threadIdx.x; i < S; i += 64) {
-                const int l = i ^ j;
-                if (l > i) {
-                    const uint32_t a = k[i], b = k[l];
-                    const bool up = (i & kk) == 0;
-                    if ((a > b) == up) {
-                        k[i] = b;
-                        k[l] = a;
+#pragma unroll
+            for (int r = 0; r < kPairs; ++r)
+                if (lane + 64 * r < half) {
+                    const bool up = (ia[r] & kk) == 0;
+                    if ((a[r] > b[r]) == up) {
+                        k[ia[r]] = b[r];
+                        k[ia[r] | j] = a[r];
                     }
                 }
-            }
             __syncthreads();
         }
 }
@@ -2210,7 +2210,7 @@ __global__ void __launch_bounds__(64) k_build_rows(uint64_t seed, int64_t n, int
         for (int w = lane; w < kDevRowMax / 32; w += 64) taken[w] = 0u;
         __syncthreads();
         // 2. first occurrences of each value
-        sort64_wave(key, S);
+        sort_row<uint64_t>(key, S);
         for (int q = lane; q < nsyn; q += 64) {
             const uint64_t kq = key[q];
             const bool f = q == 0 || (key[q - 1] >> 32) != (kq >> 32);
@@ -2267,7 +2267,7 @@ __global__ void __launch_bounds__(64) k_build_rows(uint64_t seed, int64_t n, int
             }
         }
         __syncthreads();
-        sort32_wave(val, S);
+        sort_row<uint32_t>(val, S);
         const float g = i < n_exc ? g_exc : g_inh;
         uint2 *out = tab + (size_t)(i - row0) * (size_t)nsyn;
         for (int d = lane; d < nsyn; d += 64) {

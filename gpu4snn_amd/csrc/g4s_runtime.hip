@@ -113,8 +113,9 @@ int build_synapses(const g4s_spec *spec, uint2 *syn, hipStream_t stream)
     const int64_t n_exc = n_exc_of(spec);
     if (nsyn <= kDeviceBuildMaxNsyn) {
         const float scale = 1000.0f / (float)nsyn;  // the host builder's gains
-        hipError_t e = launch_build_rows(spec->seed, n, n_exc, nsyn, scale * 0.5f, -scale, syn, 0, n, stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        // queued on `stream`; the callers wait for the stream after the
+        // instance's initialisation, which follows the build there
+        const hipError_t e = launch_build_rows(spec->seed, n, n_exc, nsyn, scale * 0.5f, -scale, syn, 0, n, stream);
         if (e != hipSuccess) return fail(G4S_ERR_HIP, "device connectivity build: %s", hipGetErrorString(e));
         return G4S_OK;
     }
