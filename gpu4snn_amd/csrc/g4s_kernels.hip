@@ -2129,153 +2129,183 @@ __global__ void __launch_bounds__(1024) k_skl_slice(DevNet d, int t0, int steps,
 // candidates -- for d = 0..Nsyn-1 with jj_d = m-Nsyn+d: t_d uniform on
 // [0, jj_d] from Philox (i, d, CONN); take t_d unless already taken, else
 // jj_d -- mapped v -> v + (v >= i), sorted, with weights g*U(Philox(i, j,
-// WEIGHT)).  Restated for a wave per row:
+// WEIGHT)).  Restated for a workgroup per row:
 //  * a value below J0 = jj_0 can only be taken as a draw, so draw d takes
-//    t_d < J0 exactly when d is its first occurrence (found by sorting the
-//    pairs (t_d, d));
+//    t_d < J0 exactly when d is the first draw of that value: the least d
+//    per value, from an LDS hash table (open addressing, at most half full);
 //  * a value jj_b >= J0 is taken either as draw b's fallback (bit b) or as
 //    a draw t_k = jj_b at k >= b; only draws t_k >= J0 read that state, so
-//    one lane resolves those draws in order over an Nsyn-bit set -- in
-//    sparse rows a handful per row (about Nsyn^2 / 2N).
-// One 64-thread workgroup per row; Nsyn <= kDevRowMax.
+//    one thread resolves those draws in ascending d over an Nsyn-bit set --
+//    in sparse rows a handful per row (about Nsyn^2 / 2N);
+//  * the row's values, distinct, are put in order by buckets of equal value
+//    ranges (a count per bucket, a scan, a place per value) and a value's
+//    rank is its bucket's start plus the values below it in its bucket.
+// kBuildThreads threads per row; Nsyn <= kDevRowMax.
 // ---------------------------------------------------------------------------
 constexpr int kDevRowMax = 1024;
+constexpr int kBuildThreads = 256;
+constexpr int kBuildPer = kDevRowMax / kBuildThreads;  // draws per thread
+constexpr uint32_t kHashEmpty = 0xFFFFFFFFu;           // no value (values are below 2^31)
 
-// Bitonic sort of k[0..S) in LDS by one 64-thread workgroup (S a power of
-// two, 64 <= S <= kDevRowMax).  A stage compares the pairs (i, i | j) with
-// bit j of i clear: pair q is i = q with a zero bit inserted at bit log2(j).
-// Lane l takes pairs l + 64r; it reads all of its pairs before it writes any
-// (the pairs of a stage are disjoint), so a stage waits for one LDS round
-// trip, not one per pair.
-template <class T>
-__device__ __forceinline__ void sort_row(T *k, int S)
+__global__ void __launch_bounds__(kBuildThreads) k_build_rows(uint64_t seed, int64_t n, int64_t n_exc, int32_t nsyn,
+                                                              float g_exc, float g_inh, uint2 *tab, int64_t row0,
+                                                              int64_t rows)
 {
-    constexpr int kPairs = kDevRowMax / 128;
-    const int half = S >> 1;
-    const int lane = (int)threadIdx.x;
-    for (int kk = 2; kk <= S; kk <<= 1)
-        for (int j = kk >> 1; j > 0; j >>= 1) {
-            T a[kPairs], b[kPairs];
-            int ia[kPairs];
-#pragma unroll
-            for (int r = 0; r < kPairs; ++r) {
-                const int q = lane + 64 * r;
-                ia[r] = ((q & ~(j - 1)) << 1) | (q & (j - 1));
-                if (q < half) {
-                    a[r] = k[ia[r]];
-                    b[r] = k[ia[r] | j];
-                }
-            }
-#pragma unroll
-            for (int r = 0; r < kPairs; ++r)
-                if (lane + 64 * r < half) {
-                    const bool up = (ia[r] & kk) == 0;
-                    if ((a[r] > b[r]) == up) {
-                        k[ia[r]] = b[r];
-                        k[ia[r] | j] = a[r];
-                    }
-                }
-            __syncthreads();
-        }
-}
-
-__global__ void __launch_bounds__(64) k_build_rows(uint64_t seed, int64_t n, int64_t n_exc, int32_t nsyn, float g_exc,
-                                                   float g_inh, uint2 *tab, int64_t row0, int64_t rows)
-{
-    __shared__ uint64_t key[kDevRowMax];
     __shared__ uint32_t tv[kDevRowMax];
     __shared__ uint32_t val[kDevRowMax];
-    __shared__ uint8_t first[kDevRowMax];
-    __shared__ uint16_t hi_list[kDevRowMax];
-    __shared__ uint32_t taken[kDevRowMax / 32];
-    const int lane = (int)threadIdx.x;
+    // the hash table (keys, least draw) until the draws are settled, then the
+    // buckets: counts and starts (kDevRowMax + 1), placed values, row order
+    __shared__ uint32_t pool[4 * kDevRowMax];
+    __shared__ uint32_t taken[kDevRowMax / 32], hib[kDevRowMax / 32];
+    __shared__ int s_scan[kBuildThreads / kWave];
+    uint32_t *hkey = pool, *hmin = pool + 2 * kDevRowMax;
+    uint32_t *bstart = pool, *bkt = pool + kDevRowMax + 32, *ord = pool + 2 * kDevRowMax + 64;
+    const int tid = (int)threadIdx.x;
     const uint64_t m = (uint64_t)(n - 1);
     const uint64_t J0 = m - (uint64_t)nsyn;
     int S = 64;
     while (S < nsyn) S <<= 1;
+    const int H = 2 * S;  // hash slots
+    const int hshift = 32 - (31 - __builtin_clz((unsigned)H));
+    // buckets: S of them, value v in bucket (v * mul) >> 32, non-decreasing in
+    // v and below S for v < n (mul = 2^32 when N <= S: one value per bucket)
+    const uint64_t mul = (uint64_t)n <= (uint64_t)S ? (1ull << 32) : (((uint64_t)S << 32) / (uint64_t)n);
+    const int words = (nsyn + 31) / 32;
     for (int64_t i = row0 + blockIdx.x; i < row0 + rows; i += gridDim.x) {
-        // 1. draws
-        for (int d = lane; d < S; d += 64) {
+        // 1. draws; the table and the bit sets cleared
+        uint32_t t[kBuildPer];
+#pragma unroll
+        for (int k = 0; k < kBuildPer; ++k) {
+            const int d = tid + kBuildThreads * k;
+            t[k] = kHashEmpty;
             if (d < nsyn) {
                 const uint64_t jj = J0 + (uint64_t)d;
                 const Word4 o = draw(seed, (uint32_t)i, (uint32_t)d, kTagConn);
                 const uint64_t r = ((uint64_t)o.x1 << 32) | o.x0;
-                const uint32_t t = (uint32_t)__umul64hi(r, jj + 1u);
-                tv[d] = t;
-                key[d] = ((uint64_t)t << 32) | (uint32_t)d;
-            } else {
-                key[d] = ~0ull;
+                t[k] = (uint32_t)__umul64hi(r, jj + 1u);
+                tv[d] = t[k];
             }
         }
-        for (int w = lane; w < kDevRowMax / 32; w += 64) taken[w] = 0u;
-        __syncthreads();
-        // 2. first occurrences of each value
-        sort_row<uint64_t>(key, S);
-        for (int q = lane; q < nsyn; q += 64) {
-            const uint64_t kq = key[q];
-            const bool f = q == 0 || (key[q - 1] >> 32) != (kq >> 32);
-            first[(uint32_t)kq] = f ? 1 : 0;
+        for (int h = tid; h < H; h += kBuildThreads) {
+            hkey[h] = kHashEmpty;
+            hmin[h] = kHashEmpty;
+        }
+        for (int w = tid; w < words; w += kBuildThreads) {
+            taken[w] = 0u;
+            hib[w] = 0u;
         }
         __syncthreads();
-        // 3. draws below J0 settle at once; fallbacks of repeated ones take
-        //    their jj; the others are listed in ascending d
-        int nhi = 0;
-        for (int base = 0; base < S; base += 64) {
-            const int d = base + lane;
-            bool hi = false;
-            if (d < nsyn) {
-                const uint32_t t = tv[d];
-                if ((uint64_t)t < J0) {
-                    if (first[d]) {
-                        val[d] = t;
-                    } else {
-                        val[d] = (uint32_t)(J0 + (uint64_t)d);
-                        atomicOr(&taken[d >> 5], 1u << (d & 31));
-                    }
-                } else {
-                    hi = true;
+        // 2. the least draw of each value below J0
+        int slot[kBuildPer];
+#pragma unroll
+        for (int k = 0; k < kBuildPer; ++k) {
+            const int d = tid + kBuildThreads * k;
+            slot[k] = -1;
+            if (d < nsyn && (uint64_t)t[k] < J0) {
+                uint32_t h = (t[k] * 0x9E3779B1u) >> hshift;
+                for (;;) {  // ends: the table is at most half full
+                    const uint32_t old = atomicCAS(&hkey[h], kHashEmpty, t[k]);
+                    if (old == kHashEmpty || old == t[k]) break;
+                    h = (h + 1u) & (uint32_t)(H - 1);
                 }
+                atomicMin(&hmin[h], (uint32_t)d);
+                slot[k] = (int)h;
             }
-            const uint64_t mask = __ballot(hi);
-            if (hi) hi_list[nhi + bits_below(mask)] = (uint16_t)d;
-            nhi += __popcll(mask);
         }
         __syncthreads();
-        // 4. draws at or above J0, in order, by one lane
-        if (lane == 0) {
-            for (int h = 0; h < nhi; ++h) {
-                const int d = hi_list[h];
-                const uint32_t b = (uint32_t)((uint64_t)tv[d] - J0);
-                const bool is_taken = (int)b == d || ((taken[b >> 5] >> (b & 31)) & 1u);
-                if (is_taken) {
+        // 3. draws below J0 settle at once (fallbacks of repeated values take
+        //    their jj); the others are marked for step 4
+#pragma unroll
+        for (int k = 0; k < kBuildPer; ++k) {
+            const int d = tid + kBuildThreads * k;
+            if (d >= nsyn) continue;
+            if (slot[k] >= 0) {
+                if (hmin[slot[k]] == (uint32_t)d) {
+                    val[d] = t[k];
+                } else {
                     val[d] = (uint32_t)(J0 + (uint64_t)d);
-                    taken[d >> 5] |= 1u << (d & 31);
-                } else {
-                    val[d] = tv[d];
-                    taken[b >> 5] |= 1u << (b & 31);
+                    atomicOr(&taken[d >> 5], 1u << (d & 31));
                 }
-            }
-        }
-        __syncthreads();
-        // 5. skip the presynaptic neuron itself, sort, weights
-        for (int d = lane; d < S; d += 64) {
-            if (d < nsyn) {
-                const uint32_t v = val[d];
-                val[d] = v + ((int64_t)v >= i ? 1u : 0u);
             } else {
-                val[d] = 0xFFFFFFFFu;
+                atomicOr(&hib[d >> 5], 1u << (d & 31));
             }
         }
         __syncthreads();
-        sort_row<uint32_t>(val, S);
+        // 4. draws at or above J0, in ascending d, by one thread; the others
+        //    clear the bucket counts (over the table, no longer read)
+        if (tid == 0) {
+            for (int w = 0; w < words; ++w)
+                for (uint32_t bits = hib[w]; bits; bits &= bits - 1u) {
+                    const int d = 32 * w + __builtin_ctz(bits);
+                    const uint32_t b = (uint32_t)((uint64_t)tv[d] - J0);
+                    const bool is_taken = (int)b == d || ((taken[b >> 5] >> (b & 31)) & 1u);
+                    if (is_taken) {
+                        val[d] = (uint32_t)(J0 + (uint64_t)d);
+                        taken[d >> 5] |= 1u << (d & 31);
+                    } else {
+                        val[d] = tv[d];
+                        taken[b >> 5] |= 1u << (b & 31);
+                    }
+                }
+        }
+        for (int q = tid; q <= S; q += kBuildThreads) bstart[q] = 0u;
+        __syncthreads();
+        // 5. skip the presynaptic neuron itself; count the buckets
+        uint32_t v[kBuildPer], bk[kBuildPer], pos[kBuildPer];
+#pragma unroll
+        for (int k = 0; k < kBuildPer; ++k) {
+            const int d = tid + kBuildThreads * k;
+            v[k] = bk[k] = pos[k] = 0u;
+            if (d < nsyn) {
+                v[k] = val[d] + ((int64_t)val[d] >= i ? 1u : 0u);
+                bk[k] = (uint32_t)(((uint64_t)v[k] * mul) >> 32);
+                pos[k] = atomicAdd(&bstart[bk[k]], 1u);
+            }
+        }
+        __syncthreads();
+        // 6. bucket starts: thread q scans buckets [q*per, (q+1)*per)
+        {
+            const int per = (S + kBuildThreads - 1) / kBuildThreads;
+            const int q0 = min(tid * per, S), q1 = min(q0 + per, S);
+            int c = 0;
+            for (int q = q0; q < q1; ++q) c += (int)bstart[q];
+            int tot = 0;
+            const int lane = lane_id(), wave = tid / kWave;
+            const int incl = wave_inclusive_sum(c);
+            if (lane == kWave - 1) s_scan[wave] = incl;
+            __syncthreads();  // every count was read above
+            int run = waves_prefix_sum(s_scan, &tot) + incl - c;
+            for (int q = q0; q < q1; ++q) {
+                const int cq = (int)bstart[q];
+                bstart[q] = (uint32_t)run;
+                run += cq;
+            }
+            if (tid == 0) bstart[S] = (uint32_t)nsyn;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kBuildPer; ++k)
+            if (tid + kBuildThreads * k < nsyn) bkt[bstart[bk[k]] + pos[k]] = v[k];
+        __syncthreads();
+        // 7. rank = bucket start + the bucket's values below this one
+#pragma unroll
+        for (int k = 0; k < kBuildPer; ++k)
+            if (tid + kBuildThreads * k < nsyn) {
+                const uint32_t b0 = bstart[bk[k]], b1 = bstart[bk[k] + 1];
+                uint32_t r = b0;
+                for (uint32_t q = b0; q < b1; ++q) r += bkt[q] < v[k] ? 1u : 0u;
+                ord[r] = v[k];
+            }
+        __syncthreads();
+        // 8. the row in order, with its weights
         const float g = i < n_exc ? g_exc : g_inh;
         uint2 *out = tab + (size_t)(i - row0) * (size_t)nsyn;
-        for (int d = lane; d < nsyn; d += 64) {
-            const uint32_t j = val[d];
+        for (int d = tid; d < nsyn; d += kBuildThreads) {
+            const uint32_t j = ord[d];
             const Word4 o = draw(seed, (uint32_t)i, j, kTagWeight);
             out[d] = make_uint2(j, f2u(g * u01(o.x0)));
         }
-        __syncthreads();
+        __syncthreads();  // the next row clears what this one still reads
     }
 }
 
@@ -2565,7 +2595,8 @@ hipError_t launch_build_rows(uint64_t seed, int64_t n, int64_t n_exc, int32_t ns
 {
     if (nsyn < 1 || nsyn > kDevRowMax || rows <= 0) return hipErrorInvalidValue;
     const int grid = (int)std::min<int64_t>(rows, 65536);
-    hipLaunchKernelGGL(k_build_rows, dim3(grid), dim3(64), 0, s, seed, n, n_exc, nsyn, g_exc, g_inh, tab, row0, rows);
+    hipLaunchKernelGGL(k_build_rows, dim3(grid), dim3(kBuildThreads), 0, s, seed, n, n_exc, nsyn, g_exc, g_inh, tab, row0,
+                       rows);
     return hipGetLastError();
 }
 
