@@ -276,8 +276,9 @@ def bench_serve(g, args):
 def extras(g, args):
     """Batched and served summaries beside the single-network line, in the
     same accumulation mode: 256 networks in one batch (1000 steps), then
-    open-loop serving at 50 req/s for 120 s and an overload run at 400 req/s
-    for 10 s."""
+    open-loop serving at 50 req/s for 120 s and an overload run at 1000 req/s
+    for 10 s (offered about 2500 simulated seconds per second, above what one
+    device serves)."""
     import copy
     t0 = time.perf_counter()
     a = copy.copy(args)
@@ -288,7 +289,7 @@ def extras(g, args):
     s1.serve, s1.rate = 120.0, 50.0
     sv = bench_serve(g, s1)
     s2 = copy.copy(a)
-    s2.serve, s2.rate = 10.0, 400.0
+    s2.serve, s2.rate = 10.0, 1000.0
     sat = bench_serve(g, s2)
     print(f"served done in {time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
     return {
@@ -299,7 +300,7 @@ def extras(g, args):
         "served": {"workload": sv["config"]["workload"], "accum": a.accum, "p50_s": sv["latency_s"]["p50"],
                    "p99_s": sv["latency_s"]["p99"], "sim_s_per_s": sv["value"],
                    "offered_sim_s_per_s": sv["config"]["offered_sim_s_per_s"],
-                   "overload": {"rate_req_s": 400.0, "sim_s_per_s": sat["value"], "p50_s": sat["latency_s"]["p50"],
+                   "overload": {"rate_req_s": s2.rate, "sim_s_per_s": sat["value"], "p50_s": sat["latency_s"]["p50"],
                                 "p99_s": sat["latency_s"]["p99"], "speedup_vs_plain": sat["speedup_vs_plain"]},
                    "plain_sim_s_per_s": sat["plain"]["sim_s_per_s"]},
     }
