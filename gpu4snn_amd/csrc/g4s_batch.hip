@@ -465,6 +465,12 @@ int batch_add(g4s_batch *b, const g4s_spec *spec, int64_t steps, int keep_raster
 }
 }  // namespace
 
+int g4s::batch_add_sliced(g4s_batch *b, const g4s_spec *spec, int64_t steps, int keep_raster, int32_t slice_len,
+                          int32_t *slot)
+{
+    return batch_add(b, spec, steps, keep_raster, slice_len, slot);
+}
+
 extern "C" {
 
 int g4s_batch_add(g4s_batch *b, const g4s_spec *spec, int64_t steps, int keep_raster, int32_t *slot)

@@ -15,6 +15,9 @@ int64_t n_exc_of(const g4s_spec *s);
 // (Nsyn <= kDeviceBuildMaxNsyn; queued on `stream`, not waited for) or on
 // the host and uploaded; same table.
 int build_synapses(const g4s_spec *spec, uint2 *syn, hipStream_t stream);
+// g4s_batch_add with the instance's slice length given (0: the batch's rule).
+int batch_add_sliced(g4s_batch *b, const g4s_spec *spec, int64_t steps, int keep_raster, int32_t slice_len,
+                     int32_t *slot);
 
 #define G4S_HIP_OK(expr)                                                                           \
     do {                                                                                           \

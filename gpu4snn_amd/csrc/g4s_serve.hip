@@ -58,6 +58,7 @@ struct g4s_server {
     std::deque<int64_t> pending;
     std::vector<int64_t> active;
     int64_t admitting = 0;  // taken from `pending`, not yet active or failed
+    bool fixed_slice = false;  // the caller chose the slice length
     int64_t next_id = 1;
     bool stop = false;
     std::thread worker;
@@ -70,6 +71,23 @@ struct g4s_server {
 };
 
 namespace {
+
+// Slice length of a request admitted now.  Unless the caller fixed it:
+// kLightSlice while fewer than kBusyActive requests are resident (a chunk's
+// steps are then latency-bound, and more slices per instance run them
+// sooner), kBusySlice beyond (requests pile up: the device is throughput-
+// bound, and longer slices look up each spike fewer times).  Measured at
+// 1000 requests per second (offered about 2540 simulated seconds per second):
+// 2198 served with 384-neuron slices, 2363 with 512, 2417 with 768.  At 400
+// per second (kept up with, about 110 resident) 384 gives p50 / p99 0.12 /
+// 0.27 s, a switch at 48 resident 0.19 / 0.38 s.
+constexpr int32_t kLightSlice = 384, kBusySlice = 768;
+constexpr size_t kBusyActive = 192;
+int32_t slice_for(const g4s_server *s)
+{
+    if (s->fixed_slice) return s->opts.slice_len;
+    return s->active.size() + (size_t)s->admitting >= kBusyActive ? kBusySlice : kLightSlice;
+}
 
 // Worker: admit, run a chunk, retire; sleeps while there is nothing to do.
 void serve_loop(g4s_server *s)
@@ -99,7 +117,7 @@ void serve_loop(g4s_server *s)
                 r = &s->reqs[id];
             }
             int32_t slot = -1;
-            const int rc = g4s_batch_add(s->batch, &r->spec, r->steps, r->keep_raster ? 1 : 0, &slot);
+            const int rc = batch_add_sliced(s->batch, &r->spec, r->steps, r->keep_raster ? 1 : 0, slice_for(s), &slot);
             std::lock_guard<std::mutex> lk(s->mu);
             r->t_admit = s->now();
             s->admitting -= 1;
@@ -185,16 +203,15 @@ int g4s_server_create(const g4s_server_opts *opts, g4s_server **out)
     if (o.budget < 0 || o.chunk < 0) return fail(G4S_ERR_ARG, "bad server options");
     g4s_batch_opts bo{};
     bo.chunk = o.chunk > 0 ? o.chunk : 100;
-    // Serving defaults to 384-neuron slices, shorter than the batch default:
-    // at 50 requests per second (the batch a fraction of the device) a
-    // request's steps are latency-bound, and more slices per instance run
-    // them faster.  Measured p50 / p99 at 50 requests per second: 0.043 /
-    // 0.122 s at 256, 0.048 / 0.120 s at 384, 0.051-0.053 / 0.131-0.139 s at
-    // 512, 0.062 / 0.156 s at 768, the same aggregate rate; at 400 requests
-    // per second (overload) 256 falls behind (1000 simulated seconds per
-    // second, p99 1.45 s) while 384 keeps up with 512 (1017 vs 1020, p99
-    // 0.92 vs 0.88 s).
-    bo.slice_len = o.slice_len > 0 ? o.slice_len : 384;
+    // Under light load serving uses 384-neuron slices, shorter than the
+    // batch default: at 50 requests per second (the batch a fraction of the
+    // device) a request's steps are latency-bound, and more slices per
+    // instance run them faster.  Measured p50 / p99 at 50 requests per
+    // second: 0.043 / 0.122 s at 256, 0.048 / 0.120 s at 384, 0.051-0.053 /
+    // 0.131-0.139 s at 512, 0.062 / 0.156 s at 768, the same aggregate rate.
+    // Under heavy load slice_for() takes longer slices.  The footprints
+    // reserved for requests are those of 384-neuron slices, the larger.
+    bo.slice_len = o.slice_len > 0 ? o.slice_len : kLightSlice;
     int rc;
     g4s_batch *b = nullptr;
     if ((rc = g4s_batch_create(o.accum, &bo, &b))) return rc;
@@ -202,6 +219,7 @@ int g4s_server_create(const g4s_server_opts *opts, g4s_server **out)
     s->opts = o;
     s->opts.chunk = bo.chunk;
     s->opts.slice_len = bo.slice_len;
+    s->fixed_slice = o.slice_len > 0;
     s->batch = b;
     s->t0 = std::chrono::steady_clock::now();
     s->worker = std::thread(serve_loop, s);
