@@ -83,10 +83,11 @@ namespace {
 // 0.27 s, a switch at 48 resident 0.19 / 0.38 s.
 constexpr int32_t kLightSlice = 384, kBusySlice = 768;
 constexpr size_t kBusyActive = 192;
-int32_t slice_for(const g4s_server *s)
+int32_t slice_for(const g4s_server *s, const g4s_spec &spec)
 {
     if (s->fixed_slice) return s->opts.slice_len;
-    return s->active.size() + (size_t)s->admitting >= kBusyActive ? kBusySlice : kLightSlice;
+    const int32_t want = s->active.size() + (size_t)s->admitting >= kBusyActive ? kBusySlice : kLightSlice;
+    return (int32_t)std::min<int64_t>(want, (spec.n + 63) / 64 * 64);
 }
 
 // Worker: admit, run a chunk, retire; sleeps while there is nothing to do.
@@ -117,7 +118,8 @@ void serve_loop(g4s_server *s)
                 r = &s->reqs[id];
             }
             int32_t slot = -1;
-            const int rc = batch_add_sliced(s->batch, &r->spec, r->steps, r->keep_raster ? 1 : 0, slice_for(s), &slot);
+            const int rc =
+                batch_add_sliced(s->batch, &r->spec, r->steps, r->keep_raster ? 1 : 0, slice_for(s, r->spec), &slot);
             std::lock_guard<std::mutex> lk(s->mu);
             r->t_admit = s->now();
             s->admitting -= 1;

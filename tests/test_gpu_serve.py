@@ -109,3 +109,28 @@ def test_release_and_states(g4s):
             srv.info(rid)
         with pytest.raises(g4s.G4SError):
             srv.submit(10, 10, 1, 5)
+
+
+def test_busy_server_det(g4s, oracle):
+    """More requests resident than the light-load limit (192): later
+    admissions take longer slices, and a second wave reuses the device
+    blocks of the first; every sampled result is the oracle's."""
+    rng = np.random.default_rng(7)
+    reqs = [(int(rng.integers(1000, 4000)), 100, 500 + k, float(rng.uniform(0.9, 1.3)), int(rng.integers(300, 400)))
+            for k in range(260)]
+    wave2 = [(int(rng.integers(1000, 4000)), 100, 900 + k, 1.1, 150) for k in range(12)]
+    with g4s.Server("det", chunk=50) as srv:
+        ids = [srv.submit(n, ns, sd, st, thal_scale=sc, keep_raster=False) for (n, ns, sd, sc, st) in reqs]
+        srv.drain(300)
+        st = srv.stats()
+        assert st["completed"] == len(reqs) and st["failed"] == 0
+        assert st["max_active"] >= 192, st
+        ids2 = [srv.submit(n, ns, sd, st2, thal_scale=sc, keep_raster=False) for (n, ns, sd, sc, st2) in wave2]
+        srv.drain(120)
+        assert srv.stats()["completed"] == len(reqs) + len(wave2)
+        sample = [(ids[k], reqs[k]) for k in (0, 1, 97, 198, 259)] + list(zip(ids2, wave2))
+        for rid, (n, ns, sd, sc, steps) in sample:
+            info = srv.info(rid)
+            assert info["state"] == "done" and info["t"] == steps
+            _, ocounts = oracle_raster(oracle, n, ns, sd, sc, steps)
+            assert np.array_equal(srv.counts(rid), ocounts), (n, sd)
