@@ -83,6 +83,8 @@ namespace {
 // 0.27 s, a switch at 48 resident 0.19 / 0.38 s.
 constexpr int32_t kLightSlice = 384, kBusySlice = 768;
 constexpr size_t kBusyActive = 192;
+// Called by the worker, the only thread that changes `active` and
+// `admitting`, so it reads them without the lock.
 int32_t slice_for(const g4s_server *s, const g4s_spec &spec)
 {
     if (s->fixed_slice) return s->opts.slice_len;
