@@ -1722,24 +1722,22 @@ __device__ __forceinline__ void slice_accumulate(const Net &d, int t, int p, int
     if (total == 0) __syncthreads();  // otherwise the last pass ended with one
 }
 
-// Offset table: thread per (row i, boundary p): first index of row i whose
-// target is >= p*L (binary search in the ascending row).
+// Offset table: soff[i*(P+1) + p] = first index of row i whose target is
+// >= min(p*L, N).  One workgroup per row (grid-stride over rows), one pass
+// over the row: entry e (targets ascending, x_{-1} = -1) is the answer for
+// every boundary p*L in (x_{e-1}, x_e], and index Nsyn for the boundaries
+// beyond the last target, up to p = P.  Each p is written once.
 __global__ void __launch_bounds__(256) k_slice_offsets(DevNet d, uint16_t *soff)
 {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    const int64_t total = (int64_t)d.n * (d.P + 1);
-    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += stride) {
-        const int64_t i = g / (d.P + 1);
-        const int p = (int)(g - i * (d.P + 1));
-        const uint32_t key = (uint32_t)min<int64_t>((int64_t)p * d.L, d.n);
-        const uint2 *row = d.syn + (size_t)i * (size_t)d.nsyn;
-        int lo = 0, hi = d.nsyn;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (row[mid].x < key) lo = mid + 1;
-            else hi = mid;
+    const int L = d.L, P = d.P, ns = d.nsyn;
+    for (int64_t i = blockIdx.x; i < (int64_t)d.n; i += gridDim.x) {
+        const uint2 *row = d.syn + (size_t)i * (size_t)ns;
+        uint16_t *out = soff + (size_t)i * (size_t)(P + 1);
+        for (int e = (int)threadIdx.x; e <= ns; e += (int)blockDim.x) {
+            const int p0 = e > 0 ? (int)(row[e - 1].x / (uint32_t)L) + 1 : 0;
+            const int p1 = e < ns ? min((int)(row[e].x / (uint32_t)L), P) : P;
+            for (int q = p0; q <= p1; ++q) out[q] = (uint16_t)e;
         }
-        soff[g] = (uint16_t)lo;
     }
 }
 
@@ -2456,8 +2454,8 @@ hipError_t launch_ap(const DevNet &d, int t, int rel, int last, int prev_count, 
 
 hipError_t launch_slice_offsets(const DevNet &d, uint16_t *soff, hipStream_t s)
 {
-    const int64_t total = (int64_t)d.n * (d.P + 1);
-    hipLaunchKernelGGL(k_slice_offsets, dim3(grid_for(total, 256)), dim3(256), 0, s, d, soff);
+    const int grid = (int)std::min<int64_t>(d.n, 65536);
+    hipLaunchKernelGGL(k_slice_offsets, dim3(grid), dim3(256), 0, s, d, soff);
     return hipGetLastError();
 }
 
