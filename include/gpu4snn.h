@@ -303,7 +303,9 @@ int g4s_batch_info(const g4s_batch *b, int32_t slot, g4s_inst_info *info);
 int g4s_batch_counts(const g4s_batch *b, int32_t slot, int32_t *counts);
 int g4s_batch_raster(const g4s_batch *b, int32_t slot, uint32_t *words);
 int g4s_batch_state(g4s_batch *b, int32_t slot, float *v, float *u, float *I, int32_t *nospks);
-/* Free an instance's device memory; its slot may be reused. */
+/* Release an instance; its slot may be reused.  Its device block is kept
+ * for later instances that fit it (up to 16 GB of such blocks per batch,
+ * the oldest freed beyond), and freed by g4s_batch_destroy. */
 int g4s_batch_remove(g4s_batch *b, int32_t slot);
 /* Timing of the last g4s_batch_step / g4s_batch_run call (launches = kernel
  * launches of the batched step kernel, grid = its largest grid). */
@@ -360,7 +362,7 @@ int g4s_batch_tune_mix(int accum, const g4s_spec *specs, int32_t count, int32_t 
  * footprint (g4s_batch_footprint) fits the budget, runs one chunk of every
  * admitted request per iteration (requests arriving meanwhile join at the
  * next chunk: continuous batching), and retires finished requests, copying
- * their counts (and raster) to host memory and freeing their device memory.
+ * their counts (and raster) to host memory and releasing their instances.
  * All functions are thread-safe.  Times are seconds since server creation.
  * ---------------------------------------------------------------------- */
 typedef struct g4s_server g4s_server;
