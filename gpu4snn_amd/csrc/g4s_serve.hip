@@ -73,22 +73,24 @@ struct g4s_server {
 namespace {
 
 // Slice length of a request admitted now.  Unless the caller fixed it:
-// kLightSlice while fewer than kBusyActive requests are resident (a chunk's
-// steps are then latency-bound, and more slices per instance run them
-// sooner), kBusySlice beyond (requests pile up: the device is throughput-
-// bound, and longer slices look up each spike fewer times).  Measured at
-// 1000 requests per second (offered about 2540 simulated seconds per second):
-// 2198 served with 384-neuron slices, 2363 with 512, 2417 with 768.  At 400
-// per second (kept up with, about 110 resident) 384 gives p50 / p99 0.12 /
-// 0.27 s, a switch at 48 resident 0.19 / 0.38 s.
-constexpr int32_t kLightSlice = 384, kBusySlice = 768;
-constexpr size_t kBusyActive = 192;
+// shorter slices while few requests are resident (a chunk's steps are then
+// latency-bound, and more slices per instance run them sooner), kBusySlice
+// once they pile up (the device is throughput-bound, and longer slices look
+// up each spike fewer times).  Measured: at 50 requests per second (about 10
+// resident) p50 / p99 0.035 / 0.105 s with 256-neuron slices, 0.039 / 0.105 s
+// with 384; at 400 per second (about 85 resident) 0.126 / 0.296 s with 256,
+// 0.121 / 0.270 s with 384, 0.19 / 0.38 s with 768 beyond 48 resident; at
+// 1000 per second (offered about 2540 simulated seconds per second) 2198
+// served with 384, 2363 with 512, 2417 with 768.
+constexpr int32_t kIdleSlice = 256, kLightSlice = 384, kBusySlice = 768;
+constexpr size_t kLightActive = 32, kBusyActive = 192;
 // Called by the worker, the only thread that changes `active` and
 // `admitting`, so it reads them without the lock.
 int32_t slice_for(const g4s_server *s, const g4s_spec &spec)
 {
     if (s->fixed_slice) return s->opts.slice_len;
-    const int32_t want = s->active.size() + (size_t)s->admitting >= kBusyActive ? kBusySlice : kLightSlice;
+    const size_t resident = s->active.size() + (size_t)s->admitting;
+    const int32_t want = resident < kLightActive ? kIdleSlice : resident < kBusyActive ? kLightSlice : kBusySlice;
     return (int32_t)std::min<int64_t>(want, (spec.n + 63) / 64 * 64);
 }
 
@@ -207,15 +209,14 @@ int g4s_server_create(const g4s_server_opts *opts, g4s_server **out)
     if (o.budget < 0 || o.chunk < 0) return fail(G4S_ERR_ARG, "bad server options");
     g4s_batch_opts bo{};
     bo.chunk = o.chunk > 0 ? o.chunk : 100;
-    // Under light load serving uses 384-neuron slices, shorter than the
-    // batch default: at 50 requests per second (the batch a fraction of the
-    // device) a request's steps are latency-bound, and more slices per
-    // instance run them faster.  Measured p50 / p99 at 50 requests per
-    // second: 0.043 / 0.122 s at 256, 0.048 / 0.120 s at 384, 0.051-0.053 /
-    // 0.131-0.139 s at 512, 0.062 / 0.156 s at 768, the same aggregate rate.
-    // Under heavy load slice_for() takes longer slices.  The footprints
-    // reserved for requests are those of 384-neuron slices, the larger.
-    bo.slice_len = o.slice_len > 0 ? o.slice_len : kLightSlice;
+    // Serving uses slices shorter than the batch default while the load is
+    // light (an earlier build at 50 requests per second, p50 / p99: 0.043 /
+    // 0.122 s at 256, 0.048 / 0.120 s at 384, 0.051-0.053 / 0.131-0.139 s
+    // at 512, 0.062 / 0.156 s at 768, the same aggregate rate).
+    // slice_for() picks each request's slice length by load; the footprints
+    // reserved for requests are those of the shortest (kIdleSlice), the
+    // largest.
+    bo.slice_len = o.slice_len > 0 ? o.slice_len : kIdleSlice;
     int rc;
     g4s_batch *b = nullptr;
     if ((rc = g4s_batch_create(o.accum, &bo, &b))) return rc;

@@ -370,8 +370,9 @@ typedef struct g4s_server g4s_server;
 typedef struct g4s_server_opts {
     int32_t accum;       /* G4S_ACCUM_* of every request                        */
     int32_t chunk;       /* steps per chunk, 0 = 100                            */
-    int32_t slice_len;   /* as g4s_batch_opts.slice_len, 0 = by load: 384 while
-                            fewer than 192 requests are resident, else 768     */
+    int32_t slice_len;   /* as g4s_batch_opts.slice_len, 0 = by load: 256 while
+                            fewer than 32 requests are resident, 384 while
+                            fewer than 192, else 768                          */
     int32_t reserved;    /* must be 0                                           */
     int64_t budget;      /* device bytes for resident requests, 0 = no limit   */
 } g4s_server_opts;
